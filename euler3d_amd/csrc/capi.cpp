@@ -1,0 +1,766 @@
+// C ABI of libe3d_hip.so: mesh and configuration handling on the host, device context that keeps
+// one case resident in HBM and advances it with the kernels of kernels.hip.
+#include "../../include/e3d.h"
+#include "device.hpp"
+#include "e3d_host.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_lastError;
+
+int fail(int code, const std::string &msg) {
+	g_lastError = msg;
+	return code;
+}
+
+struct HipError : std::runtime_error {
+	using std::runtime_error::runtime_error;
+};
+
+void hipCheck(hipError_t e, const char *what) {
+	if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template<typename F>
+int guarded(F &&f) {
+	try {
+		return f();
+	} catch (const HipError &e) {
+		return fail(E3D_ERR_DEVICE, e.what());
+	} catch (const std::invalid_argument &e) {
+		return fail(E3D_ERR_ARG, e.what());
+	} catch (const std::exception &e) {
+		return fail(E3D_ERR_MESH, e.what());
+	}
+}
+
+template<typename T>
+struct DBuf {
+	T *p = nullptr;
+	size_t n = 0;
+	void alloc(size_t count) {
+		free();
+		n = count;
+		if (count) hipCheck(hipMalloc((void **) &p, count * sizeof(T)), "hipMalloc");
+	}
+	void upload(const T *h, size_t count, hipStream_t s) {
+		if (count) hipCheck(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+	}
+	void free() {
+		if (p) (void) hipFree(p);
+		p = nullptr;
+		n = 0;
+	}
+	~DBuf() { free(); }
+};
+
+}// namespace
+
+struct e3d_mesh {
+	e3d::MeshFile file;
+	e3d::SolverMesh sm;
+	bool preprocessed = false;
+};
+
+struct e3d_ctx {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	e3d_params prm{};
+	e3d::DevFree fr{};
+	e3d::DevMesh dm{};
+	int nElem = 0, nFace = 0, nFaceInt = 0, nGhost = 0, nTot = 0;
+	int machDerived = 0;
+	double lastMs = 0.0;
+	// per-launch timing of the stage kernel(s)
+	bool timing = false;
+	std::vector<hipEvent_t> evPool;
+	size_t evUsed = 0;
+	// mesh
+	DBuf<int> e0, e1, fstart, flist;
+	DBuf<double> nx, ny, nz, S, vol, sx, sy, sz;
+	DBuf<int8_t> gkind;
+	// state
+	DBuf<double> st[8]; // rho u v w p E H M
+	DBuf<double> W0, fs, partial, scalars;
+	// aero (wall list order)
+	int nWall = 0;
+	DBuf<int> wallCell;
+	DBuf<double> wN[3], wC[3];
+	double cpFactor = 0.0;
+
+	e3d::DevState state() {
+		return {st[0].p, st[1].p, st[2].p, st[3].p, st[4].p, st[5].p, st[6].p, st[7].p};
+	}
+	void sync() { hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+	void check() { hipCheck(hipGetLastError(), "kernel launch"); }
+};
+
+namespace {
+
+void fillFree(const e3d_params &p, e3d::DevFree &q) {
+	// FlowField::Initialize, FlowField.cpp:72-81 (rho_inf = p_inf = 1)
+	const double pi = 3.141592653589793238;
+	double g = p.gamma;
+	q.gamma = g;
+	q.cfl = p.cfl;
+	q.rho_inf = 1.0;
+	q.p_inf = 1.0;
+	q.M_inf = p.mach;
+	q.u_inf = q.M_inf * std::sqrt(g) * std::cos(p.aoa_deg * (pi / 180));
+	q.v_inf = q.M_inf * std::sqrt(g) * std::sin(p.aoa_deg * (pi / 180));
+	q.w_inf = 0.0;
+	q.E_inf = q.p_inf / ((g - 1) * q.rho_inf) + ((q.u_inf * q.u_inf + q.v_inf * q.v_inf + q.w_inf * q.w_inf) / 2.0);
+	q.H_inf = q.E_inf + (q.p_inf / q.rho_inf);
+}
+
+void initFreestream(e3d_ctx *c) {
+	std::vector<double> h(c->nTot);
+	const double vals[8] = {c->fr.rho_inf, c->fr.u_inf, c->fr.v_inf, c->fr.w_inf, c->fr.p_inf, c->fr.E_inf, c->fr.H_inf, c->fr.M_inf};
+	for (int k = 0; k < 8; k++) {
+		std::fill(h.begin(), h.end(), vals[k]);
+		c->st[k].upload(h.data(), c->nTot, c->stream);
+	}
+	c->sync();
+	c->machDerived = 0;
+}
+
+void runStage(e3d_ctx *c, int stage, double *rmsPartial) {
+	static const double alphas[5] = {0.0533, 0.1263, 0.2375, 0.4414, 1.0};
+	auto s = c->state();
+	hipEvent_t evA = nullptr, evB = nullptr;
+	if (c->timing) {
+		if (c->evUsed + 2 > c->evPool.size()) {
+			for (int i = 0; i < 64; i++) {
+				hipEvent_t e;
+				hipCheck(hipEventCreate(&e), "hipEventCreate");
+				c->evPool.push_back(e);
+			}
+		}
+		evA = c->evPool[c->evUsed++];
+		evB = c->evPool[c->evUsed++];
+		hipCheck(hipEventRecord(evA, c->stream), "hipEventRecord");
+	}
+	e3d::launchBc(c->dm, s, c->fr, c->machDerived, c->stream);
+	e3d::launchFlux(c->dm, s, c->fr, c->prm.scheme, c->fs.p, nullptr, c->stream);
+	e3d::launchStage(c->dm, s, c->fr, c->fs.p, c->W0.p, alphas[stage], stage == 0, rmsPartial, nullptr, c->stream);
+	if (c->timing) hipCheck(hipEventRecord(evB, c->stream), "hipEventRecord");
+	c->machDerived = 1;
+}
+
+const char *kStageKernelName = "kBc+kFlux+kStage";
+
+// one iteration of Run's body; the sum of R_rho^2 of the last stage lands in *dsum (device)
+void runIteration(e3d_ctx *c, double *dsum) {
+	for (int stage = 0; stage < 5; stage++) runStage(c, stage, stage == 4 ? c->partial.p : nullptr);
+	e3d::launchSum(c->partial.p, e3d::stageBlocks(c->nElem), dsum, c->stream);
+}
+
+}// namespace
+
+extern "C" {
+
+const char *e3d_last_error(void) { return g_lastError.c_str(); }
+const char *e3d_version(void) { return "e3d-mi355x 0.1 (gfx950, FP64 plain path)"; }
+
+int e3d_params_default(e3d_params *out) {
+	if (!out) return fail(E3D_ERR_ARG, "null params");
+	e3d::Params p;
+	std::memset(out, 0, sizeof(*out));
+	out->mach = 0.5;
+	out->aoa_deg = 0.0;
+	out->gamma = p.gamma;
+	out->cfl = 1.5;
+	out->sref = 1.0;
+	out->cref = 1.0;
+	out->orient_cl[0] = 1, out->orient_cl[1] = 1;
+	out->orient_cd[0] = 0, out->orient_cd[1] = 1;
+	out->orient_cm[0] = 2, out->orient_cm[1] = 1;
+	out->scheme = 0;
+	out->time_integration = 2;
+	out->residual_smoothing = 0;
+	out->max_iter = 1;
+	out->sampling = 1;
+	out->min_residual = 0.0;
+	out->min_aerocoeff_error = 0.0;
+	return E3D_OK;
+}
+
+int e3d_config_read(const char *path, e3d_params *out, char *mesh_path, int mesh_path_len, char *out_dir,
+                    int out_dir_len) {
+	if (!path || !out) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		e3d::Params p = e3d::readConfig(path);
+		std::memset(out, 0, sizeof(*out));
+		out->mach = p.mach;
+		out->aoa_deg = p.aoa;
+		out->gamma = p.gamma;
+		out->cfl = p.cfl;
+		out->sref = p.sref;
+		out->cref = p.cref;
+		for (int k = 0; k < 3; k++) out->ref_point[k] = p.refPoint[k];
+		for (int k = 0; k < 2; k++) {
+			out->orient_cl[k] = p.orientCL[k];
+			out->orient_cd[k] = p.orientCD[k];
+			out->orient_cm[k] = p.orientCM[k];
+		}
+		out->scheme = p.scheme;
+		out->time_integration = p.timeIntegration;
+		out->residual_smoothing = p.residualSmoothing;
+		out->max_iter = p.maxIter;
+		out->sampling = p.sampling;
+		out->min_residual = p.minResidual;
+		out->min_aerocoeff_error = p.minAeroCoeffError;
+		if (p.nPartition != 1)
+			return fail(E3D_ERR_UNSUPPORTED, "PARTITION_FILES must be 1: the whole mesh is resident on one device");
+		auto copy = [](const std::string &s, char *dst, int len) {
+			if (!dst || len <= 0) return;
+			std::strncpy(dst, s.c_str(), (size_t) len - 1);
+			dst[len - 1] = 0;
+		};
+		copy(p.partitionFiles.empty() ? std::string() : p.partitionFiles[0], mesh_path, mesh_path_len);
+		copy(p.outputDir, out_dir, out_dir_len);
+		return E3D_OK;
+	});
+}
+
+int e3d_mesh_read(const char *path, int require_par, e3d_mesh **out) {
+	if (!path || !out) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		auto m = std::make_unique<e3d_mesh>();
+		try {
+			m->file = e3d::readSu2(path, require_par != 0);
+		} catch (const std::exception &e) {
+			return fail(E3D_ERR_IO, e.what());
+		}
+		*out = m.release();
+		return E3D_OK;
+	});
+}
+
+int e3d_mesh_write(const e3d_mesh *m, const char *path, int precision, int par_block) {
+	if (!m || !path) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		e3d::writeSu2(m->file, path, precision, par_block != 0);
+		return E3D_OK;
+	});
+}
+
+int e3d_mesh_gen_box(int nx, int ny, int nz, double lx, double ly, double lz, const char *tags6, e3d_mesh **out) {
+	if (!tags6 || !out) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		std::vector<std::string> t;
+		std::string s(tags6), cur;
+		for (char ch : s) {
+			if (ch == ',') {
+				t.push_back(cur);
+				cur.clear();
+			} else cur += ch;
+		}
+		t.push_back(cur);
+		if (t.size() != 6) return fail(E3D_ERR_ARG, "tags6 needs six comma-separated tags");
+		const char *tp[6];
+		for (int i = 0; i < 6; i++) tp[i] = t[i].c_str();
+		auto m = std::make_unique<e3d_mesh>();
+		m->file = e3d::genBox(nx, ny, nz, lx, ly, lz, tp);
+		*out = m.release();
+		return E3D_OK;
+	});
+}
+
+int e3d_mesh_gen_naca(int ni, int nj, int nk, double thickness, double camber, double camber_pos, double wall_spacing,
+                      double farfield_radius, double dz, e3d_mesh **out) {
+	if (!out) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		auto m = std::make_unique<e3d_mesh>();
+		m->file = e3d::genNacaOgrid(ni, nj, nk, thickness, camber, camber_pos, wall_spacing, farfield_radius, dz, 0);
+		*out = m.release();
+		return E3D_OK;
+	});
+}
+
+int e3d_mesh_gen_bump(int nx, int ny, int nz, double jitter, unsigned long long seed, int permute, e3d_mesh **out) {
+	if (!out) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		auto m = std::make_unique<e3d_mesh>();
+		m->file = e3d::genBumpTets(nx, ny, nz, jitter, seed, permute != 0);
+		*out = m.release();
+		return E3D_OK;
+	});
+}
+
+int e3d_mesh_gen_wing(int ni, int nj, int nk, double thickness, double sweep_deg, double taper, double semispan,
+                      double farfield, e3d_mesh **out) {
+	if (!out) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		auto m = std::make_unique<e3d_mesh>();
+		m->file = e3d::genWing(ni, nj, nk, thickness, sweep_deg, taper, semispan, farfield);
+		*out = m.release();
+		return E3D_OK;
+	});
+}
+
+int e3d_mesh_preprocess(e3d_mesh *m, double cref) {
+	if (!m) return fail(E3D_ERR_ARG, "null mesh");
+	if (!(cref > 0)) return fail(E3D_ERR_ARG, "CREF must be positive");
+	return guarded([&] {
+		m->sm = e3d::preprocess(m->file, cref);
+		m->preprocessed = true;
+		return E3D_OK;
+	});
+}
+
+int e3d_mesh_get_info(const e3d_mesh *m, e3d_mesh_info *out) {
+	if (!m || !out) return fail(E3D_ERR_ARG, "null argument");
+	out->n_node = m->file.nNode();
+	out->n_elem = m->file.nElem();
+	out->n_face = m->sm.nFace;
+	out->n_face_int = m->sm.nFaceInt;
+	out->n_ghost = m->sm.nGhost;
+	out->n_wall = (int) m->sm.wallGhost.size();
+	out->n_far = (int) m->sm.farGhost.size();
+	out->n_sym = (int) m->sm.symGhost.size();
+	return E3D_OK;
+}
+
+int e3d_mesh_get_i32(const e3d_mesh *m, const char *name, int *out, long long capacity, long long *n) {
+	if (!m || !name) return fail(E3D_ERR_ARG, "null argument");
+	const std::vector<int> *v = nullptr;
+	std::string s(name);
+	const auto &sm = m->sm;
+	if (s == "face2element") v = &sm.face2element;
+	else if (s == "face2nodeStart") v = &sm.face2nodeStart;
+	else if (s == "face2node") v = &sm.face2node;
+	else if (s == "element2faceStart") v = &sm.element2faceStart;
+	else if (s == "element2face") v = &sm.element2face;
+	else if (s == "element2element") v = &sm.element2element;
+	else if (s == "wallGhost") v = &sm.wallGhost;
+	else if (s == "wallCell") v = &sm.wallCell;
+	else if (s == "wallFace") v = &sm.wallFace;
+	else if (s == "farGhost") v = &sm.farGhost;
+	else if (s == "farCell") v = &sm.farCell;
+	else if (s == "farFace") v = &sm.farFace;
+	else if (s == "symGhost") v = &sm.symGhost;
+	else if (s == "symCell") v = &sm.symCell;
+	else if (s == "symFace") v = &sm.symFace;
+	else if (s == "ghostKind") v = &sm.ghostKind;
+	else if (s == "elemVtk") v = &m->file.elemVtk;
+	else if (s == "elemStart") v = &m->file.elemStart;
+	else if (s == "elemNodes") v = &m->file.elemNodes;
+	else return fail(E3D_ERR_ARG, "unknown int32 array " + s);
+	if (n) *n = (long long) v->size();
+	if (out) std::memcpy(out, v->data(), sizeof(int) * (size_t) std::min<long long>(capacity, (long long) v->size()));
+	return E3D_OK;
+}
+
+int e3d_mesh_get_f64(const e3d_mesh *m, const char *name, double *out, long long capacity, long long *n) {
+	if (!m || !name) return fail(E3D_ERR_ARG, "null argument");
+	const std::vector<double> *v = nullptr;
+	std::string s(name);
+	const auto &sm = m->sm;
+	if (s == "faceNormals") v = &sm.faceNormal;
+	else if (s == "faceNormalsUnit") v = &sm.faceUnitNormal;
+	else if (s == "faceSurfaces") v = &sm.faceArea;
+	else if (s == "faceCenters") v = &sm.faceCenter;
+	else if (s == "cellVolumes") v = &sm.cellVolume;
+	else if (s == "cellCentroids") v = &sm.cellCentroid;
+	else if (s == "xyz") v = &m->file.xyz;
+	else return fail(E3D_ERR_ARG, "unknown f64 array " + s);
+	if (n) *n = (long long) v->size();
+	if (out) std::memcpy(out, v->data(), sizeof(double) * (size_t) std::min<long long>(capacity, (long long) v->size()));
+	return E3D_OK;
+}
+
+void e3d_mesh_free(e3d_mesh *m) { delete m; }
+
+int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out) {
+	if (!m || !p || !out) return fail(E3D_ERR_ARG, "null argument");
+	if (!m->preprocessed) return fail(E3D_ERR_ARG, "mesh is not preprocessed (call e3d_mesh_preprocess)");
+	if (p->time_integration != 2)
+		return fail(E3D_ERR_UNSUPPORTED, "only TIME_INTEGRATION= 2 (5-stage Runge-Kutta) is on the device path");
+	if (p->residual_smoothing) return fail(E3D_ERR_UNSUPPORTED, "RESIDUAL_SMOOTHING is not on the device path");
+	if (p->scheme != 0 && p->scheme != 1) return fail(E3D_ERR_ARG, "SCHEME must be 0 (Roe) or 1 (AUSM)");
+	if (!(p->cfl > 0) || !(p->sampling > 0) || p->max_iter < 0) return fail(E3D_ERR_ARG, "invalid CFL/SAMPLING/MAX_ITER");
+	const e3d::SolverMesh &sm = m->sm;
+	if ((long long) sm.nElem + sm.nGhost >= (1LL << 31) || (long long) sm.nFace >= (1LL << 31))
+		return fail(E3D_ERR_ARG, "mesh too large for 32-bit indexing");
+	return guarded([&] {
+		auto c = std::make_unique<e3d_ctx>();
+		int nDev = 0;
+		hipCheck(hipGetDeviceCount(&nDev), "hipGetDeviceCount");
+		if (device < 0 || device >= nDev) return fail(E3D_ERR_DEVICE, "no HIP device " + std::to_string(device));
+		c->device = device;
+		hipCheck(hipSetDevice(device), "hipSetDevice");
+		hipCheck(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+		hipCheck(hipEventCreate(&c->ev0), "hipEventCreate");
+		hipCheck(hipEventCreate(&c->ev1), "hipEventCreate");
+		c->prm = *p;
+		fillFree(*p, c->fr);
+		c->nElem = sm.nElem;
+		c->nFace = sm.nFace;
+		c->nFaceInt = sm.nFaceInt;
+		c->nGhost = sm.nGhost;
+		c->nTot = sm.nElem + sm.nGhost;
+		const size_t nF = sm.nFace, nE = sm.nElem;
+		hipStream_t s = c->stream;
+
+		// faces: SoA
+		std::vector<int> he0(nF), he1(nF);
+		std::vector<double> hx(nF), hy(nF), hz(nF);
+		for (size_t f = 0; f < nF; f++) {
+			he0[f] = sm.face2element[2 * f];
+			he1[f] = sm.face2element[2 * f + 1];
+			hx[f] = sm.faceUnitNormal[3 * f];
+			hy[f] = sm.faceUnitNormal[3 * f + 1];
+			hz[f] = sm.faceUnitNormal[3 * f + 2];
+		}
+		c->e0.alloc(nF), c->e0.upload(he0.data(), nF, s);
+		c->e1.alloc(nF), c->e1.upload(he1.data(), nF, s);
+		c->nx.alloc(nF), c->nx.upload(hx.data(), nF, s);
+		c->ny.alloc(nF), c->ny.upload(hy.data(), nF, s);
+		c->nz.alloc(nF), c->nz.upload(hz.data(), nF, s);
+		c->S.alloc(nF), c->S.upload(sm.faceArea.data(), nF, s);
+		c->sync();
+
+		// ghosts
+		std::vector<int8_t> gk(sm.nGhost);
+		for (int k = 0; k < sm.nGhost; k++) gk[k] = (int8_t) (sm.ghostKind[k] == e3d::BND_WALL ? 1 : sm.ghostKind[k] == e3d::BND_SYM ? 2 : 0);
+		c->gkind.alloc(sm.nGhost), c->gkind.upload(gk.data(), sm.nGhost, s);
+
+		// cells: static time-step sums in local-face order, face CSR in ascending face id
+		std::vector<double> hsx(nE), hsy(nE), hsz(nE);
+		std::vector<int> hstart(nE + 1), hlist(sm.element2face.size());
+		for (size_t e = 0; e < nE; e++) {
+			double ax = 0, ay = 0, az = 0;
+			int b = sm.element2faceStart[e], en = sm.element2faceStart[e + 1];
+			for (int j = b; j < en; j++) {
+				int f = sm.element2face[j];
+				ax += std::fabs(sm.faceNormal[3 * (size_t) f]);
+				ay += std::fabs(sm.faceNormal[3 * (size_t) f + 1]);
+				az += std::fabs(sm.faceNormal[3 * (size_t) f + 2]);
+			}
+			hsx[e] = ax, hsy[e] = ay, hsz[e] = az;
+			hstart[e] = b;
+			std::vector<int> fl(sm.element2face.begin() + b, sm.element2face.begin() + en);
+			std::sort(fl.begin(), fl.end());
+			for (int j = 0; j < en - b; j++) {
+				int f = fl[j];
+				bool second = sm.face2element[2 * (size_t) f + 1] == (int) e && sm.face2element[2 * (size_t) f] != (int) e;
+				hlist[b + j] = second ? (int) ((unsigned) f | 0x80000000u) : f;
+			}
+		}
+		hstart[nE] = (int) sm.element2face.size();
+		c->vol.alloc(nE), c->vol.upload(sm.cellVolume.data(), nE, s);
+		c->sx.alloc(nE), c->sx.upload(hsx.data(), nE, s);
+		c->sy.alloc(nE), c->sy.upload(hsy.data(), nE, s);
+		c->sz.alloc(nE), c->sz.upload(hsz.data(), nE, s);
+		c->fstart.alloc(nE + 1), c->fstart.upload(hstart.data(), nE + 1, s);
+		c->flist.alloc(hlist.size()), c->flist.upload(hlist.data(), hlist.size(), s);
+		c->sync();
+
+		// wall list in the reference's order for the force integration
+		c->nWall = (int) sm.wallGhost.size();
+		if (c->nWall) {
+			std::vector<double> h[6];
+			for (auto &v : h) v.resize(c->nWall);
+			for (int i = 0; i < c->nWall; i++) {
+				int f = sm.wallFace[i];
+				for (int k = 0; k < 3; k++) {
+					h[k][i] = sm.faceNormal[3 * (size_t) f + k];
+					h[3 + k][i] = sm.faceCenter[3 * (size_t) f + k];
+				}
+			}
+			c->wallCell.alloc(c->nWall), c->wallCell.upload(sm.wallCell.data(), c->nWall, s);
+			for (int k = 0; k < 3; k++) {
+				c->wN[k].alloc(c->nWall), c->wN[k].upload(h[k].data(), c->nWall, s);
+				c->wC[k].alloc(c->nWall), c->wC[k].upload(h[3 + k].data(), c->nWall, s);
+			}
+			c->sync();
+		}
+		double uinf2 = c->fr.u_inf * c->fr.u_inf + c->fr.v_inf * c->fr.v_inf + c->fr.w_inf * c->fr.w_inf;
+		c->cpFactor = 1 / (0.5 * c->fr.rho_inf * uinf2);
+
+		for (auto &b : c->st) b.alloc(c->nTot);
+		c->W0.alloc(5 * nE);
+		c->fs.alloc(5 * nF);
+		c->partial.alloc(std::max(1, e3d::stageBlocks(c->nElem)));
+		c->scalars.alloc(16);
+
+		c->dm.nElem = c->nElem;
+		c->dm.nFace = c->nFace;
+		c->dm.nFaceInt = c->nFaceInt;
+		c->dm.nGhost = c->nGhost;
+		c->dm.f = {c->e0.p, c->e1.p, c->nx.p, c->ny.p, c->nz.p, c->S.p};
+		c->dm.c = {c->vol.p, c->sx.p, c->sy.p, c->sz.p, c->fstart.p, c->flist.p};
+		c->dm.gkind = c->gkind.p;
+		initFreestream(c.get());
+		*out = c.release();
+		return E3D_OK;
+	});
+}
+
+int e3d_set_state(e3d_ctx *c, const double *const fields[8]) {
+	if (!c || !fields) return fail(E3D_ERR_ARG, "null argument");
+	for (int k = 0; k < 8; k++)
+		if (!fields[k]) return fail(E3D_ERR_ARG, "null state field");
+	return guarded([&] {
+		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		for (int k = 0; k < 8; k++) c->st[k].upload(fields[k], c->nTot, c->stream);
+		c->sync();
+		c->machDerived = 0;
+		return E3D_OK;
+	});
+}
+
+int e3d_get_state(e3d_ctx *c, double *const fields[8]) {
+	if (!c || !fields) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		if (c->machDerived) {
+			e3d::launchMach(c->dm, c->state(), c->fr, c->stream);
+			c->check();
+		}
+		for (int k = 0; k < 8; k++)
+			if (fields[k])
+				hipCheck(hipMemcpyAsync(fields[k], c->st[k].p, sizeof(double) * c->nTot, hipMemcpyDeviceToHost, c->stream),
+				         "hipMemcpyAsync D2H");
+		c->sync();
+		return E3D_OK;
+	});
+}
+
+int e3d_update_bc(e3d_ctx *c) {
+	if (!c) return fail(E3D_ERR_ARG, "null context");
+	return guarded([&] {
+		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		e3d::launchBc(c->dm, c->state(), c->fr, c->machDerived, c->stream);
+		c->check();
+		c->sync();
+		return E3D_OK;
+	});
+}
+
+namespace {
+void toComponentMinor(const std::vector<double> &soa, size_t n, double *out) {
+	for (size_t i = 0; i < n; i++)
+		for (int k = 0; k < 5; k++) out[5 * i + k] = soa[k * n + i];
+}
+}// namespace
+
+int e3d_face_flux(e3d_ctx *c, double *F, double *FS) {
+	if (!c) return fail(E3D_ERR_ARG, "null context");
+	return guarded([&] {
+		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		DBuf<double> raw;
+		raw.alloc(5 * (size_t) c->nFace);
+		e3d::launchFlux(c->dm, c->state(), c->fr, c->prm.scheme, c->fs.p, raw.p, c->stream);
+		c->check();
+		std::vector<double> h(5 * (size_t) c->nFace);
+		if (F) {
+			hipCheck(hipMemcpyAsync(h.data(), raw.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream), "D2H");
+			c->sync();
+			toComponentMinor(h, c->nFace, F);
+		}
+		if (FS) {
+			hipCheck(hipMemcpyAsync(h.data(), c->fs.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream), "D2H");
+			c->sync();
+			toComponentMinor(h, c->nFace, FS);
+		}
+		c->sync();
+		return E3D_OK;
+	});
+}
+
+int e3d_residual(e3d_ctx *c, double *R) {
+	if (!c || !R) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		DBuf<double> dR;
+		dR.alloc(5 * (size_t) c->nElem);
+		e3d::launchFlux(c->dm, c->state(), c->fr, c->prm.scheme, c->fs.p, nullptr, c->stream);
+		e3d::launchGather(c->dm, c->fs.p, dR.p, c->stream);
+		c->check();
+		std::vector<double> h(5 * (size_t) c->nElem);
+		hipCheck(hipMemcpyAsync(h.data(), dR.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream), "D2H");
+		c->sync();
+		toComponentMinor(h, c->nElem, R);
+		return E3D_OK;
+	});
+}
+
+int e3d_timestep(e3d_ctx *c, double *dt) {
+	if (!c || !dt) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		DBuf<double> d;
+		d.alloc(c->nElem);
+		e3d::launchTimestep(c->dm, c->state(), c->fr, d.p, c->stream);
+		c->check();
+		hipCheck(hipMemcpyAsync(dt, d.p, sizeof(double) * c->nElem, hipMemcpyDeviceToHost, c->stream), "D2H");
+		c->sync();
+		return E3D_OK;
+	});
+}
+
+int e3d_iterate(e3d_ctx *c, int n, double *rms_sum) {
+	if (!c || n < 0) return fail(E3D_ERR_ARG, "bad argument");
+	return guarded([&] {
+		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		DBuf<double> sums;
+		sums.alloc(std::max(1, n));
+		hipCheck(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
+		for (int it = 0; it < n; it++) runIteration(c, sums.p + it);
+		hipCheck(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
+		c->check();
+		c->sync();
+		float ms = 0;
+		hipCheck(hipEventElapsedTime(&ms, c->ev0, c->ev1), "hipEventElapsedTime");
+		c->lastMs = ms;
+		if (rms_sum && n > 0)
+			hipCheck(hipMemcpy(rms_sum, sums.p, sizeof(double) * n, hipMemcpyDeviceToHost), "D2H");
+		return E3D_OK;
+	});
+}
+
+int e3d_aero(e3d_ctx *c, double cf[3], double cm[3]) {
+	if (!c || !cf || !cm) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		double h[6] = {0, 0, 0, 0, 0, 0};
+		if (c->nWall) {
+			e3d::launchAero(c->nWall, c->wallCell.p, c->wN[0].p, c->wN[1].p, c->wN[2].p, c->wC[0].p, c->wC[1].p, c->wC[2].p,
+			                c->st[4].p, c->cpFactor, c->fr.p_inf, c->prm.cref, c->prm.sref, c->scalars.p, c->stream);
+			c->check();
+			hipCheck(hipMemcpyAsync(h, c->scalars.p, sizeof(h), hipMemcpyDeviceToHost, c->stream), "D2H");
+			c->sync();
+		}
+		// AeroCoefficients.cpp:49-51: lever = ref - CP (CP never divided), M = F x lever / Cref
+		double lever[3] = {c->prm.ref_point[0] - h[3], c->prm.ref_point[1] - h[4], c->prm.ref_point[2] - h[5]};
+		double M[3] = {h[1] * lever[2] - h[2] * lever[1], h[2] * lever[0] - h[0] * lever[2], h[0] * lever[1] - h[1] * lever[0]};
+		for (int k = 0; k < 3; k++) {
+			cf[k] = h[k];
+			cm[k] = M[k] / c->prm.cref;
+		}
+		return E3D_OK;
+	});
+}
+
+int e3d_run(e3d_ctx *c, double *rows, int max_rows, int *n_rows, int *iters, double coeffs[3], double cfOut[3],
+            double cmOut[3]) {
+	if (!c) return fail(E3D_ERR_ARG, "null context");
+	return guarded([&] {
+		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		const e3d_params &q = c->prm;
+		int nb = 0, counter = 0, nrows = 0;
+		double CL = 0, CD = 0, CM = 0, cf[3] = {0, 0, 0}, cm[3] = {0, 0, 0};
+		auto row = [&](int it, double rms) {
+			CL = cf[q.orient_cl[0]] * q.orient_cl[1];
+			CD = cf[q.orient_cd[0]] * q.orient_cd[1];
+			CM = cm[q.orient_cm[0]] * q.orient_cm[1];
+			if (rows && nrows < max_rows) {
+				double *r = rows + 8 * (size_t) nrows;
+				r[0] = it, r[1] = rms;
+				for (int k = 0; k < 3; k++) r[2 + k] = cf[k], r[5 + k] = cm[k];
+			}
+			nrows++;
+		};
+		double *dsum = c->scalars.p + 8;
+		hipCheck(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
+		while (nb < q.max_iter) {
+			runIteration(c, dsum);
+			c->check();
+			double sum = 0;
+			hipCheck(hipMemcpyAsync(&sum, dsum, sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+			c->sync();
+			double rms = std::sqrt(sum / c->nElem);
+			if (rms < q.min_residual) {
+				row(nb, rms);
+				break;
+			}
+			nb += 1;
+			if (nb % q.sampling == 0) {
+				double oCL = CL, oCD = CD, oCM = CM;
+				int rc = e3d_aero(c, cf, cm);
+				if (rc != E3D_OK) return rc;
+				row(nb, rms);
+				double tol = q.min_aerocoeff_error;
+				bool crit = std::fabs(oCL - CL) < tol && std::fabs(oCD - CD) < tol && std::fabs(oCM - CM) < tol;
+				counter = crit ? counter + 1 : 0;
+				if (counter >= 2) break;
+			}
+		}
+		hipCheck(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
+		c->sync();
+		float ms = 0;
+		hipCheck(hipEventElapsedTime(&ms, c->ev0, c->ev1), "hipEventElapsedTime");
+		c->lastMs = ms;
+		if (n_rows) *n_rows = nrows;
+		if (iters) *iters = nb;
+		if (coeffs) coeffs[0] = CL, coeffs[1] = CD, coeffs[2] = CM;
+		for (int k = 0; k < 3; k++) {
+			if (cfOut) cfOut[k] = cf[k];
+			if (cmOut) cmOut[k] = cm[k];
+		}
+		return E3D_OK;
+	});
+}
+
+int e3d_last_elapsed_ms(e3d_ctx *c, double *ms) {
+	if (!c || !ms) return fail(E3D_ERR_ARG, "null argument");
+	*ms = c->lastMs;
+	return E3D_OK;
+}
+
+int e3d_set_kernel_timing(e3d_ctx *c, int enable) {
+	if (!c) return fail(E3D_ERR_ARG, "null context");
+	c->timing = enable != 0;
+	c->evUsed = 0;
+	return E3D_OK;
+}
+
+int e3d_kernel_timing(e3d_ctx *c, double *avg_ms, int *launches, char *name, int name_len) {
+	if (!c) return fail(E3D_ERR_ARG, "null context");
+	return guarded([&] {
+		c->sync();
+		double total = 0.0;
+		int n = (int) (c->evUsed / 2);
+		for (int i = 0; i < n; i++) {
+			float ms = 0;
+			hipCheck(hipEventElapsedTime(&ms, c->evPool[2 * i], c->evPool[2 * i + 1]), "hipEventElapsedTime");
+			total += ms;
+		}
+		if (avg_ms) *avg_ms = n ? total / n : 0.0;
+		if (launches) *launches = n;
+		if (name && name_len > 0) {
+			std::strncpy(name, kStageKernelName, (size_t) name_len - 1);
+			name[name_len - 1] = 0;
+		}
+		c->evUsed = 0;
+		return E3D_OK;
+	});
+}
+
+void e3d_destroy(e3d_ctx *c) {
+	if (!c) return;
+	(void) hipSetDevice(c->device);
+	if (c->stream) (void) hipStreamSynchronize(c->stream);
+	for (auto &b : c->st) b.free();
+	c->W0.free(), c->fs.free(), c->partial.free(), c->scalars.free();
+	if (c->ev0) (void) hipEventDestroy(c->ev0);
+	if (c->ev1) (void) hipEventDestroy(c->ev1);
+	for (auto e : c->evPool) (void) hipEventDestroy(e);
+	if (c->stream) (void) hipStreamDestroy(c->stream);
+	delete c;
+}
+
+}// extern "C"

@@ -1,0 +1,155 @@
+// Host-side data model of the explicit Euler solver: mesh as read from SU2/.par, the face/cell
+// structure-of-arrays derived from it, the .e3d solver parameters, and the mesh generators.
+//
+// Reference counterparts (file:line in the Euler3D tree):
+//   mesh file model     src/parser/SU2MeshParser.cpp:32-240, src/parser/MeshPartition.cpp:39-112
+//   connectivity        src/connectivity/Connectivity.cpp:70-262, include/connectivity/Connectivity.hpp:37-45
+//   ghost classification include/mesh/Mesh.hpp:471-707
+//   metrics             src/mesh/Metrics.cpp:9-309
+//   solver parameters   src/parser/SimConfig.cpp:49-236
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace e3d {
+
+// VTK cell / face type ids used by SU2 files.
+enum : int { VTK_TRI = 5, VTK_QUAD = 9, VTK_TET = 10, VTK_HEX = 12, VTK_WEDGE = 13, VTK_PYRAMID = 14 };
+
+// Boundary kind of a ghost cell. The numbering is this library's own.
+enum BoundaryKind : int { BND_FARFIELD = 0, BND_WALL = 1, BND_SYM = 2, BND_OTHER = 3 };
+
+struct Marker {
+	std::string tag;
+	std::vector<int> vtk;       // per boundary element
+	std::vector<int> nodeStart; // CSR into nodes, size n+1
+	std::vector<int> nodes;
+};
+
+// The mesh as a file holds it (SU2 / .par, one partition).
+struct MeshFile {
+	std::vector<double> xyz;       // 3 * nNode
+	std::vector<int> elemVtk;      // nElem
+	std::vector<int> elemStart;    // nElem + 1
+	std::vector<int> elemNodes;
+	std::vector<Marker> markers;
+	int nInternalMarker = 0;       // .par NINTERNAL_MARKER (must be 0 here: one partition)
+	int nNode() const { return (int) xyz.size() / 3; }
+	int nElem() const { return (int) elemVtk.size(); }
+	int nBoundaryElems() const {
+		int n = 0;
+		for (auto &m : markers) n += (int) m.vtk.size();
+		return n;
+	}
+};
+
+// Derived face/cell structure in the reference's numbering (face ids, ghost ids, orientation).
+struct SolverMesh {
+	int nElem = 0;     // interior cells
+	int nFace = 0;
+	int nFaceInt = 0;  // interior faces are ids [0, nFaceInt), boundary faces [nFaceInt, nFace)
+	int nGhost = 0;    // ghost k <-> boundary face nFaceInt + k, ghost cell id nElem + k
+
+	std::vector<int> face2element;      // 2 * nFace: (elem0, elem1); elem1 >= nElem is a ghost
+	std::vector<int> face2nodeStart;    // nFace + 1
+	std::vector<int> face2node;
+	std::vector<int> element2faceStart; // nElem + 1
+	std::vector<int> element2face;      // local-face order of each cell
+	std::vector<int> element2element;
+
+	// ghost lists in the reference's marker order (Mesh.hpp:594-707)
+	std::vector<int> wallGhost, wallCell, wallFace;
+	std::vector<int> farGhost, farCell, farFace;
+	std::vector<int> symGhost, symCell, symFace;
+	std::vector<int> ghostKind;         // nGhost, BoundaryKind of ghost nElem + k
+
+	// metrics (FP64), non-dimensionalised by CREF
+	std::vector<double> faceNormal;     // 3 * nFace, area-weighted, oriented elem0 -> elem1
+	std::vector<double> faceUnitNormal; // 3 * nFace
+	std::vector<double> faceArea;       // nFace
+	std::vector<double> faceCenter;     // 3 * nFace
+	std::vector<double> cellVolume;     // nElem
+	std::vector<double> cellCentroid;   // 3 * nElem
+};
+
+// Solver parameters from the .e3d file (SimConfig), with explicit defaults for the keys the
+// reference leaves uninitialised when absent (include/parser/SimConfig.hpp:149-179).
+struct Params {
+	std::string initialMesh;
+	int nPartition = 1;
+	std::vector<std::string> partitionFiles;
+	int orientCL[2] = {1, 1};
+	int orientCD[2] = {0, 1};
+	int orientCM[2] = {2, 1};
+	double refPoint[3] = {0, 0, 0};
+	double sref = 1.0;
+	double cref = 1.0;
+	int sampling = 1;
+	int speedOption = 0;
+	double mach = 0.0;
+	double velocity = 0.0;
+	double aoa = 0.0;
+	double pressure = 101325.0;
+	double temperature = 288.15;
+	double viscosity = 1.853e-5;
+	double density = 1.225;
+	double gamma = 1.4;
+	double gasConstant = 287.058;
+	double specificHeat = 1004.7;
+	int scheme = 0;          // 0 Roe, 1 AUSM
+	int timeIntegration = 2; // 0 explicit Euler, 1 implicit Euler, 2 RK5
+	int residualSmoothing = 0;
+	double cfl = 1.0;
+	double minResidual = 0.0;
+	double minAeroCoeffError = 0.0;
+	int maxIter = 1;
+	std::string solverLog;
+	std::string outputFile;
+	std::string outputDir;
+	std::string preLog, postLog;
+};
+
+// ---- mesh_io.cpp
+MeshFile readSu2(const std::string &path, bool requireParBlock);
+void writeSu2(const MeshFile &m, const std::string &path, int precision, bool parBlock);
+
+// ---- config.cpp
+Params readConfig(const std::string &path);
+
+// ---- preprocess.cpp
+// Builds SolverMesh from a MeshFile. O(N log N) in the number of cells; numbering, orientation
+// and every metric follow the reference exactly.
+SolverMesh preprocess(const MeshFile &m, double cref);
+
+// local faces of a cell type (Connectivity.hpp:37-45): count, and the cell-local node indices of
+// face f written to out (returns the node count)
+int localFaceCount(int vtk);
+int localFaceNodes(int vtk, int f, int out[4]);
+
+// ---- generators.cpp
+struct GenInfo {
+	double sref = 1.0, cref = 1.0;
+};
+MeshFile genBox(int nx, int ny, int nz, double lx, double ly, double lz, const char *tagsXYZ6[6]);
+MeshFile genNacaOgrid(int ni, int nj, int nk, double thicknessFactor, double camber, double camberPos,
+                      double wallSpacing, double farfieldRadius, double dz, uint64_t seed_unused);
+MeshFile genBumpTets(int nx, int ny, int nz, double jitter, uint64_t seed, bool permute);
+MeshFile genWing(int ni, int nj, int nk, double thickness, double sweepDeg, double taper, double semispan,
+                 double farfield);
+
+// splitmix64 (Steele, Lea & Flood 2014) — deterministic across compilers, unlike std distributions.
+struct SplitMix64 {
+	uint64_t s;
+	explicit SplitMix64(uint64_t seed) : s(seed) {}
+	uint64_t next() {
+		uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+		z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+		z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+		return z ^ (z >> 31);
+	}
+	double uniform() { return (double) (next() >> 11) * 0x1.0p-53; }
+};
+
+}// namespace e3d

@@ -1,0 +1,417 @@
+// HIP kernels of the explicit FP64 solver path for gfx950 (wave64).
+//
+// Arithmetic follows the reference expression by expression (file:line given per function) and
+// the file is compiled with -ffp-contract=off, so every add and multiply rounds as the
+// reference's does; FP64 division and sqrt are the IEEE correctly rounded sequences.
+// Assembly is a per-cell gather over a face CSR in ascending face id, the reference's
+// accumulation order, so the result is deterministic and bit-identical to the sequential face
+// loop without atomics.
+#include "device.hpp"
+
+namespace e3d {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct Prim {
+	double rho, u, v, w, p, H;
+};
+
+__device__ __forceinline__ Prim loadPrim(const DevState &s, int i) {
+	return {s.rho[i], s.u[i], s.v[i], s.w[i], s.p[i], s.H[i]};
+}
+
+// Solver::Roe, ConvectiveFluxesSchemes.cpp:10-165
+__device__ __forceinline__ void roeFlux(const Prim &L, const Prim &R, double nx, double ny, double nz, double gamma,
+                                        double out[5]) {
+	double sL = sqrt(L.rho), sR = sqrt(R.rho);
+	double rhoT = sqrt(L.rho * R.rho);
+	double uT = (L.u * sL + R.u * sR) / (sL + sR);
+	double vT = (L.v * sL + R.v * sR) / (sL + sR);
+	double wT = (L.w * sL + R.w * sR) / (sL + sR);
+	double hT = (L.H * sL + R.H * sR) / (sL + sR);
+	double q2 = uT * uT + vT * vT + wT * wT;
+	double cT = sqrt((gamma - 1.0) * (hT - q2 / 2));
+	double VT = uT * nx + vT * ny + wT * nz;
+	double VL = L.u * nx + L.v * ny + L.w * nz;
+	double VR = R.u * nx + R.v * ny + R.w * nz;
+	double dRho = R.rho - L.rho, dU = R.u - L.u, dV = R.v - L.v, dW = R.w - L.w, dP = R.p - L.p, dVc = VR - VL;
+	double avg[5];
+	avg[0] = (R.rho * VR + L.rho * VL) * 0.5;
+	avg[1] = ((R.rho * VR * R.u + R.p * nx) + (L.rho * VL * L.u + L.p * nx)) * 0.5;
+	avg[2] = ((R.rho * VR * R.v + R.p * ny) + (L.rho * VL * L.v + L.p * ny)) * 0.5;
+	avg[3] = ((R.rho * VR * R.w + R.p * nz) + (L.rho * VL * L.w + L.p * nz)) * 0.5;
+	avg[4] = (R.rho * VR * R.H + L.rho * VL * L.H) * 0.5;
+	double localC = sqrt(gamma * L.p / L.rho);
+	double delta = (1.0 / 15.0) * localC;
+	double a1 = VT - cT, a5 = VT + cT;
+	double l1 = fabs(a1) > delta ? fabs(a1) : (a1 * a1 + delta * delta) / (2.0 * delta);
+	double l5 = fabs(a5) > delta ? fabs(a5) : (a5 * a5 + delta * delta) / (2.0 * delta);
+	double c2x2 = 2.0 * (cT * cT);
+	double k1 = dP - rhoT * cT * dVc;
+	double k5 = dP + rhoT * cT * dVc;
+	double kr = dRho - dP / (cT * cT);
+	double aV = fabs(VT);
+	double e1[5] = {1.0, uT - cT * nx, vT - cT * ny, wT - cT * nz, hT - cT * VT};
+	double e2[5] = {1.0, uT, vT, wT, q2 * 0.5};
+	double t2[5] = {0.0, rhoT * (dU - dVc * nx), rhoT * (dV - dVc * ny), rhoT * (dW - dVc * nz),
+	                rhoT * (uT * dU + vT * dV + wT * dW - VT * dVc)};
+	double e5[5] = {1.0, uT + cT * nx, vT + cT * ny, wT + cT * nz, hT + cT * VT};
+#pragma unroll
+	for (int i = 0; i < 5; i++) {
+		double f1 = e1[i] * l1 * k1 / c2x2;
+		double f234 = (e2[i] * kr + t2[i]) * aV;
+		double f5 = e5[i] * l5 * k5 / c2x2;
+		out[i] = avg[i] - 0.5 * (f1 + f234 + f5);
+	}
+}
+
+// AUSM, Liou & Steffen, J. Comput. Phys. 107 (1993) 23-39 (the reference only declares it,
+// ConvectiveFluxesSchemes.h:27-30)
+__device__ __forceinline__ double machPlus(double M) { return fabs(M) <= 1.0 ? 0.25 * (M + 1.0) * (M + 1.0) : 0.5 * (M + fabs(M)); }
+__device__ __forceinline__ double machMinus(double M) { return fabs(M) <= 1.0 ? -0.25 * (M - 1.0) * (M - 1.0) : 0.5 * (M - fabs(M)); }
+__device__ __forceinline__ double presPlus(double M) {
+	return fabs(M) <= 1.0 ? 0.25 * (M + 1.0) * (M + 1.0) * (2.0 - M) : (M > 0.0 ? 1.0 : 0.0);
+}
+__device__ __forceinline__ double presMinus(double M) {
+	return fabs(M) <= 1.0 ? 0.25 * (M - 1.0) * (M - 1.0) * (2.0 + M) : (M < 0.0 ? 1.0 : 0.0);
+}
+
+__device__ __forceinline__ void ausmFlux(const Prim &L, const Prim &R, double nx, double ny, double nz, double g,
+                                         double out[5]) {
+	double aL = sqrt(g * L.p / L.rho), aR = sqrt(g * R.p / R.rho);
+	double VL = L.u * nx + L.v * ny + L.w * nz;
+	double VR = R.u * nx + R.v * ny + R.w * nz;
+	double ML = VL / aL, MR = VR / aR;
+	double mh = machPlus(ML) + machMinus(MR);
+	double ph = presPlus(ML) * L.p + presMinus(MR) * R.p;
+	double phiL[5] = {L.rho * aL, L.rho * aL * L.u, L.rho * aL * L.v, L.rho * aL * L.w, L.rho * aL * L.H};
+	double phiR[5] = {R.rho * aR, R.rho * aR * R.u, R.rho * aR * R.v, R.rho * aR * R.w, R.rho * aR * R.H};
+	double pn[5] = {0.0, ph * nx, ph * ny, ph * nz, 0.0};
+#pragma unroll
+	for (int i = 0; i < 5; i++) out[i] = mh * 0.5 * (phiL[i] + phiR[i]) - 0.5 * fabs(mh) * (phiR[i] - phiL[i]) + pn[i];
+}
+
+__device__ __forceinline__ void storeGhost(DevState &s, int g, double rho, double u, double v, double w, double M,
+                                           double p, double E, double H) {
+	s.rho[g] = rho;
+	s.u[g] = u;
+	s.v[g] = v;
+	s.w[g] = w;
+	s.M[g] = M;
+	s.p[g] = p;
+	s.E[g] = E;
+	s.H[g] = H;
+}
+
+// interior Mach number as FlowField::updateWRungeKutta stores it (FlowField.cpp:157, c/|u|)
+__device__ __forceinline__ double machOf(const DevState &s, int i, double g) {
+	double u = s.u[i], v = s.v[i], w = s.w[i];
+	return sqrt(g * (s.p[i] / s.rho[i])) / sqrt((u * u + v * v + w * w));
+}
+
+// EulerSolver::updateBC (EulerSolver.cpp:201-247) and PhysicalBC.cpp:11-231, one thread per ghost
+__global__ void __launch_bounds__(kBlock) kBc(DevMesh m, DevState s, DevFree q, int machDerived) {
+	int k = blockIdx.x * kBlock + threadIdx.x;
+	if (k >= m.nGhost) return;
+	int f = m.nFaceInt + k;
+	int g = m.nElem + k;
+	int i = m.f.e0[f];
+	double nx = m.f.nx[f], ny = m.f.ny[f], nz = m.f.nz[f];
+	double gm = q.gamma;
+	int kind = m.gkind[k];
+	if (kind == 0) {
+		double Vg = s.u[g] * nx + s.v[g] * ny + s.w[g] * nz;
+		double Mg = s.M[g];
+		if (Mg >= 1.0 && Vg > 0.0) {
+			double Mi = machDerived ? machOf(s, i, gm) : s.M[i];
+			storeGhost(s, g, s.rho[i], s.u[i], s.v[i], s.w[i], Mi, s.p[i], s.E[i], s.H[i]);
+		} else if (Mg < 1.0 && Vg > 0.0) {
+			double ri = s.rho[i], pi = s.p[i];
+			double c = sqrt(gm * (pi / ri));
+			double p = q.p_inf;
+			double rho = ri + (p - pi) / (c * c);
+			double u = s.u[i] + nx * (pi - p) / (ri * c);
+			double v = s.v[i] + ny * (pi - p) / (ri * c);
+			double w = s.w[i] + nz * (pi - p) / (ri * c);
+			double M = sqrt(u * u + v * v + w * w) / c;
+			double E = p / ((gm - 1) * rho) + ((u * u + v * v + w * w) / 2);
+			double H = E + (p / rho);
+			storeGhost(s, g, rho, u, v, w, M, p, E, H);
+		} else if (Mg >= 1.0 && Vg <= 0.0) {
+			storeGhost(s, g, q.rho_inf, q.u_inf, q.v_inf, q.w_inf, q.M_inf, q.p_inf, q.E_inf, q.H_inf);
+		} else if (Mg < 1.0 && Vg <= 0.0) {
+			double ri = s.rho[i], pi = s.p[i];
+			double c = sqrt(gm * (pi / ri));
+			double p = 0.5 * (q.p_inf + pi - ri * c * ((nx * (q.u_inf - s.u[i])) + (ny * (q.v_inf - s.v[i])) + (nz * (q.w_inf - s.w[i]))));
+			double rho = q.rho_inf + (p - q.p_inf) / (c * c);
+			double u = q.u_inf - nx * (q.p_inf - p) / (ri * c);
+			double v = q.v_inf - ny * (q.p_inf - p) / (ri * c);
+			double w = q.w_inf - nz * (q.p_inf - p) / (ri * c);
+			double M = sqrt(u * u + v * v + w * w) / c;
+			double E = p / ((gm - 1) * rho) + ((u * u + v * v + w * w) / 2);
+			double H = E + (p / rho);
+			storeGhost(s, g, rho, u, v, w, M, p, E, H);
+		}
+	} else {
+		// wall (PhysicalBC.cpp:124-162) and symmetry (:165-231) mirror the interior velocity
+		double ri = s.rho[i], pi = s.p[i], ui = s.u[i], vi = s.v[i], wi = s.w[i];
+		double c = sqrt(gm * (pi / ri));
+		double V = ui * nx + vi * ny + wi * nz;
+		double u = ui - 2 * V * nx;
+		double v = vi - 2 * V * ny;
+		double w = wi - 2 * V * nz;
+		double M = sqrt(u * u + v * v + w * w) / c;
+		double E = pi / ((gm - 1) * ri) + ((u * u + v * v + w * w) / 2.0);
+		double H = E + (pi / ri);
+		storeGhost(s, g, ri, u, v, w, M, pi, E, H);
+	}
+}
+
+// per-face flux as EulerSolver::computeResidual dispatches it (EulerSolver.cpp:306-358)
+__global__ void __launch_bounds__(kBlock) kFlux(DevMesh m, DevState s, DevFree q, int scheme, double *fs, double *raw) {
+	int f = blockIdx.x * kBlock + threadIdx.x;
+	if (f >= m.nFace) return;
+	int a = m.f.e0[f], b = m.f.e1[f];
+	double nx = m.f.nx[f], ny = m.f.ny[f], nz = m.f.nz[f];
+	double F[5];
+	int kind = f < m.nFaceInt ? 3 : (int) m.gkind[f - m.nFaceInt];
+	if (kind == 3 || kind == 2) {
+		Prim L = loadPrim(s, a), R = loadPrim(s, b);
+		if (scheme == 1) ausmFlux(L, R, nx, ny, nz, q.gamma, F);
+		else roeFlux(L, R, nx, ny, nz, q.gamma, F);
+	} else if (kind == 1) {
+		double pg = s.p[b];
+		F[0] = 0;
+		F[1] = nx * pg;
+		F[2] = ny * pg;
+		F[3] = nz * pg;
+		F[4] = 0;
+	} else {
+		// Solver::Fc with V = ghost velocity . n (ConvectiveFluxesSchemes.cpp:167-187)
+		double rho = s.rho[b], u = s.u[b], v = s.v[b], w = s.w[b], H = s.H[b], p = s.p[b];
+		double V = u * nx + v * ny + w * nz;
+		F[0] = rho * V;
+		F[1] = rho * u * V + p * nx;
+		F[2] = rho * v * V + p * ny;
+		F[3] = rho * w * V + p * nz;
+		F[4] = rho * H * V;
+	}
+	double S = m.f.S[f];
+	size_t nF = (size_t) m.nFace;
+#pragma unroll
+	for (int k = 0; k < 5; k++) fs[k * nF + f] = F[k] * S;
+	if (raw) {
+#pragma unroll
+		for (int k = 0; k < 5; k++) raw[k * nF + f] = F[k];
+	}
+}
+
+__device__ __forceinline__ void gatherCell(const DevMesh &m, const double *fs, int e, double r[5]) {
+	size_t nF = (size_t) m.nFace;
+#pragma unroll
+	for (int k = 0; k < 5; k++) r[k] = 0.0;
+	int b = m.c.fstart[e], en = m.c.fstart[e + 1];
+	for (int j = b; j < en; j++) {
+		int code = m.c.flist[j];
+		int f = code & 0x7fffffff;
+		if (code < 0) {
+#pragma unroll
+			for (int k = 0; k < 5; k++) r[k] -= fs[k * nF + f];
+		} else {
+#pragma unroll
+			for (int k = 0; k < 5; k++) r[k] += fs[k * nF + f];
+		}
+	}
+}
+
+__global__ void __launch_bounds__(kBlock) kGather(DevMesh m, const double *fs, double *R) {
+	int e = blockIdx.x * kBlock + threadIdx.x;
+	if (e >= m.nElem) return;
+	double r[5];
+	gatherCell(m, fs, e, r);
+	size_t n = (size_t) m.nElem;
+#pragma unroll
+	for (int k = 0; k < 5; k++) R[k * n + e] = r[k];
+}
+
+// TimeStepMethod2, TimeIntegration.cpp:8-40 (sum|N| over the cell's faces is static: precomputed
+// on the host in the same local-face order)
+__device__ __forceinline__ double localDt(const DevMesh &m, const DevFree &q, double rho, double u, double v, double w,
+                                          double p, int e) {
+	double c = sqrt(q.gamma * p / rho);
+	double lx = 0.5 * (fabs(u) + c) * m.c.sx[e];
+	double ly = 0.5 * (fabs(v) + c) * m.c.sy[e];
+	double lz = 0.5 * (fabs(w) + c) * m.c.sz[e];
+	return q.cfl * m.c.vol[e] / (lx + ly + lz);
+}
+
+__global__ void __launch_bounds__(kBlock) kTimestep(DevMesh m, DevState s, DevFree q, double *dt) {
+	int e = blockIdx.x * kBlock + threadIdx.x;
+	if (e >= m.nElem) return;
+	dt[e] = localDt(m, q, s.rho[e], s.u[e], s.v[e], s.w[e], s.p[e], e);
+}
+
+// deterministic block reduction of one double per thread (fixed tree)
+__device__ __forceinline__ double blockSum(double x, double *lds) {
+	lds[threadIdx.x] = x;
+	__syncthreads();
+	for (int off = kBlock / 2; off > 0; off >>= 1) {
+		if ((int) threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+		__syncthreads();
+	}
+	return lds[0];
+}
+
+// One RK stage for one cell: EulerSolver.cpp:431-449 / 459-470 and FlowField.cpp:140-158
+__global__ void __launch_bounds__(kBlock) kStage(DevMesh m, DevState s, DevFree q, const double *fs, double *W0, double alpha,
+                                                 int firstStage, double *rmsPartial, double *Rout) {
+	__shared__ double lds[kBlock];
+	int e = blockIdx.x * kBlock + threadIdx.x;
+	double r2 = 0.0;
+	if (e < m.nElem) {
+		size_t n = (size_t) m.nElem;
+		double r[5];
+		gatherCell(m, fs, e, r);
+		if (Rout) {
+#pragma unroll
+			for (int k = 0; k < 5; k++) Rout[k * n + e] = r[k];
+		}
+		double rho = s.rho[e], u = s.u[e], v = s.v[e], w = s.w[e], p = s.p[e];
+		double w0[5];
+		if (firstStage) {
+			double E = s.E[e];
+			w0[0] = rho;
+			w0[1] = u * rho;
+			w0[2] = v * rho;
+			w0[3] = w * rho;
+			w0[4] = E * rho;
+#pragma unroll
+			for (int k = 0; k < 5; k++) W0[k * n + e] = w0[k];
+		} else {
+#pragma unroll
+			for (int k = 0; k < 5; k++) w0[k] = W0[k * n + e];
+		}
+		double dt = localDt(m, q, rho, u, v, w, p, e);
+		double V = m.c.vol[e];
+		double nr = w0[0] - alpha * dt * r[0] / V;
+		double nru = w0[1] - alpha * dt * r[1] / V;
+		double nrv = w0[2] - alpha * dt * r[2] / V;
+		double nrw = w0[3] - alpha * dt * r[3] / V;
+		double nrE = w0[4] - alpha * dt * r[4] / V;
+		double nu = nru / nr, nv = nrv / nr, nw = nrw / nr, nE = nrE / nr;
+		double g = q.gamma;
+		double np = (g - 1) * nr * (nE - (nu * nu + nv * nv + nw * nw) / 2.0);
+		double nH = nE + (np / nr);
+		s.rho[e] = nr;
+		s.u[e] = nu;
+		s.v[e] = nv;
+		s.w[e] = nw;
+		s.E[e] = nE;
+		s.p[e] = np;
+		s.H[e] = nH;
+		r2 = r[0] * r[0];
+	}
+	if (rmsPartial) {
+		double t = blockSum(r2, lds);
+		if (threadIdx.x == 0) rmsPartial[blockIdx.x] = t;
+	}
+}
+
+__global__ void __launch_bounds__(kBlock) kSum(const double *in, int n, double *out) {
+	__shared__ double lds[kBlock];
+	double acc = 0.0;
+	// contiguous chunk per thread, summed in index order
+	int per = (n + kBlock - 1) / kBlock;
+	int b = threadIdx.x * per, en = min(n, b + per);
+	for (int i = b; i < en; i++) acc += in[i];
+	double t = blockSum(acc, lds);
+	if (threadIdx.x == 0) out[0] = t;
+}
+
+__global__ void __launch_bounds__(kBlock) kMach(DevMesh m, DevState s, DevFree q) {
+	int e = blockIdx.x * kBlock + threadIdx.x;
+	if (e >= m.nElem) return;
+	s.M[e] = machOf(s, e, q.gamma);
+}
+
+// AeroCoefficients::Update (AeroCoefficients.cpp:38-52): per wall face coeff = N*cp*Cref/Sref and
+// coeff*x_face; summed per thread over a contiguous chunk, then a fixed tree
+__global__ void __launch_bounds__(kBlock) kAero(int nWall, const int *wallCell, const double *Nx, const double *Ny,
+                                                const double *Nz, const double *Cx, const double *Cy, const double *Cz,
+                                                const double *p, double cpFactor, double pInf, double cref, double sref,
+                                                double *out6) {
+	__shared__ double lds[kBlock];
+	double acc[6] = {0, 0, 0, 0, 0, 0};
+	int per = (nWall + kBlock - 1) / kBlock;
+	int b = threadIdx.x * per, en = min(nWall, b + per);
+	for (int i = b; i < en; i++) {
+		double cp = cpFactor * (p[wallCell[i]] - pInf);
+		double cx = Nx[i] * cp * cref / sref;
+		double cy = Ny[i] * cp * cref / sref;
+		double cz = Nz[i] * cp * cref / sref;
+		acc[0] += cx;
+		acc[1] += cy;
+		acc[2] += cz;
+		acc[3] += cx * Cx[i];
+		acc[4] += cy * Cy[i];
+		acc[5] += cz * Cz[i];
+	}
+	for (int k = 0; k < 6; k++) {
+		double t = blockSum(acc[k], lds);
+		if (threadIdx.x == 0) out6[k] = t;
+		__syncthreads();
+	}
+}
+
+inline int blocksFor(int n) { return (n + kBlock - 1) / kBlock; }
+
+}// namespace
+
+int stageBlocks(int nElem) { return blocksFor(nElem); }
+
+void launchBc(const DevMesh &m, DevState s, const DevFree &q, int machDerived, hipStream_t st) {
+	if (m.nGhost == 0) return;
+	hipLaunchKernelGGL(kBc, dim3(blocksFor(m.nGhost)), dim3(kBlock), 0, st, m, s, q, machDerived);
+}
+
+void launchFlux(const DevMesh &m, DevState s, const DevFree &q, int scheme, double *fs, double *raw, hipStream_t st) {
+	if (m.nFace == 0) return;
+	hipLaunchKernelGGL(kFlux, dim3(blocksFor(m.nFace)), dim3(kBlock), 0, st, m, s, q, scheme, fs, raw);
+}
+
+void launchGather(const DevMesh &m, const double *fs, double *R, hipStream_t st) {
+	if (m.nElem == 0) return;
+	hipLaunchKernelGGL(kGather, dim3(blocksFor(m.nElem)), dim3(kBlock), 0, st, m, fs, R);
+}
+
+void launchTimestep(const DevMesh &m, DevState s, const DevFree &q, double *dt, hipStream_t st) {
+	if (m.nElem == 0) return;
+	hipLaunchKernelGGL(kTimestep, dim3(blocksFor(m.nElem)), dim3(kBlock), 0, st, m, s, q, dt);
+}
+
+void launchStage(const DevMesh &m, DevState s, const DevFree &q, const double *fs, double *W0, double alpha,
+                 int firstStage, double *rmsPartial, double *Rout, hipStream_t st) {
+	if (m.nElem == 0) return;
+	hipLaunchKernelGGL(kStage, dim3(blocksFor(m.nElem)), dim3(kBlock), 0, st, m, s, q, fs, W0, alpha, firstStage,
+	                   rmsPartial, Rout);
+}
+
+void launchSum(const double *in, int n, double *out, hipStream_t st) {
+	hipLaunchKernelGGL(kSum, dim3(1), dim3(kBlock), 0, st, in, n, out);
+}
+
+void launchMach(const DevMesh &m, DevState s, const DevFree &q, hipStream_t st) {
+	if (m.nElem == 0) return;
+	hipLaunchKernelGGL(kMach, dim3(blocksFor(m.nElem)), dim3(kBlock), 0, st, m, s, q);
+}
+
+void launchAero(int nWall, const int *wallCell, const double *Nx, const double *Ny, const double *Nz, const double *Cx,
+                const double *Cy, const double *Cz, const double *p, double cpFactor, double pInf, double cref,
+                double sref, double *out6, hipStream_t st) {
+	hipLaunchKernelGGL(kAero, dim3(1), dim3(kBlock), 0, st, nWall, wallCell, Nx, Ny, Nz, Cx, Cy, Cz, p, cpFactor, pInf,
+	                   cref, sref, out6);
+}
+
+}// namespace e3d

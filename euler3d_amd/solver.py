@@ -1,0 +1,235 @@
+"""Host-side mirror of the reference's solver interface over the C ABI.
+
+Reference seam (Euler3D):
+    SimConfig(file)                       src/parser/SimConfig.cpp:15-236       -> SimConfig
+    Mesh<MeshPartition>(file).solveConnectivity(); Metrics(mesh, rank, config)
+                                          include/mesh/Mesh.hpp:30-105, src/mesh/Metrics.cpp:9-56 -> Mesh
+    FlowField(config, mesh) + EulerSolver(...).Run()
+                                          src/solver/FlowField.cpp:8-101, src/solver/EulerSolver.cpp:21-167 -> EulerSolver
+
+Errors raise native.E3DError (the reference exits the process instead). Every compute call runs
+on the HIP device through libe3d_hip.so; nothing here computes on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import native as N
+
+STATE_FIELDS = ("rho", "u", "v", "w", "p", "E", "H", "M")
+
+I32_ARRAYS = ("face2element", "face2nodeStart", "face2node", "element2faceStart", "element2face",
+              "element2element", "wallGhost", "wallCell", "wallFace", "farGhost", "farCell", "farFace",
+              "symGhost", "symCell", "symFace", "ghostKind", "elemVtk", "elemStart", "elemNodes")
+F64_ARRAYS = ("faceNormals", "faceNormalsUnit", "faceSurfaces", "faceCenters", "cellVolumes",
+              "cellCentroids", "xyz")
+
+
+class Mesh:
+    """A mesh read from SU2/.par or generated, and (after preprocess) its faces, ghosts and metrics."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+        self.cref = None
+
+    # ---- construction
+    @classmethod
+    def read(cls, path: str, par: bool = True) -> "Mesh":
+        h = C.c_void_p()
+        N.check(N.lib().e3d_mesh_read(str(path).encode(), int(par), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def box(cls, nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, tags=("farfield",) * 6) -> "Mesh":
+        h = C.c_void_p()
+        N.check(N.lib().e3d_mesh_gen_box(nx, ny, nz, lx, ly, lz, ",".join(tags).encode(), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def naca(cls, ni=129, nj=129, nk=2, thickness=0.12, camber=0.0, camber_pos=0.0, wall_spacing=1e-3,
+             farfield_radius=20.0, dz=0.1) -> "Mesh":
+        h = C.c_void_p()
+        N.check(N.lib().e3d_mesh_gen_naca(ni, nj, nk, thickness, camber, camber_pos, wall_spacing,
+                                          farfield_radius, dz, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def bump(cls, nx=256, ny=64, nz=64, jitter=0.1, seed=0, permute=True) -> "Mesh":
+        h = C.c_void_p()
+        N.check(N.lib().e3d_mesh_gen_bump(nx, ny, nz, jitter, seed, int(permute), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def wing(cls, ni=193, nj=97, nk=65, thickness=0.10, sweep_deg=30.0, taper=0.56, semispan=1.5,
+             farfield=15.0) -> "Mesh":
+        h = C.c_void_p()
+        N.check(N.lib().e3d_mesh_gen_wing(ni, nj, nk, thickness, sweep_deg, taper, semispan, farfield,
+                                          C.byref(h)))
+        return cls(h.value)
+
+    # ---- operations
+    def write(self, path: str, precision: int = 16, par: bool = True) -> None:
+        N.check(N.lib().e3d_mesh_write(self._h, str(path).encode(), precision, int(par)))
+
+    def preprocess(self, cref: float = 1.0) -> "Mesh":
+        N.check(N.lib().e3d_mesh_preprocess(self._h, float(cref)))
+        self.cref = cref
+        return self
+
+    def info(self) -> dict:
+        m = N.MeshInfo()
+        N.check(N.lib().e3d_mesh_get_info(self._h, C.byref(m)))
+        return {k: getattr(m, k) for k, _ in N.MeshInfo._fields_}
+
+    def get(self, name: str) -> np.ndarray:
+        n = C.c_longlong()
+        if name in I32_ARRAYS:
+            N.check(N.lib().e3d_mesh_get_i32(self._h, name.encode(), None, 0, C.byref(n)))
+            out = np.empty(n.value, dtype=np.int32)
+            N.check(N.lib().e3d_mesh_get_i32(self._h, name.encode(), N.iptr(out), n.value, C.byref(n)))
+            return out
+        if name in F64_ARRAYS:
+            N.check(N.lib().e3d_mesh_get_f64(self._h, name.encode(), None, 0, C.byref(n)))
+            out = np.empty(n.value, dtype=np.float64)
+            N.check(N.lib().e3d_mesh_get_f64(self._h, name.encode(), N.dptr(out), n.value, C.byref(n)))
+            return out
+        raise KeyError(name)
+
+    def arrays(self) -> dict:
+        return {k: self.get(k) for k in I32_ARRAYS + F64_ARRAYS}
+
+    def close(self) -> None:
+        if self._h:
+            N.lib().e3d_mesh_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class SimConfig:
+    """Parsed .e3d file (SimConfig): solver parameters, the partition file and the output dir."""
+    params: N.Params
+    mesh_path: str
+    output_dir: str
+
+    @classmethod
+    def read(cls, path: str) -> "SimConfig":
+        p = N.Params()
+        mesh = C.create_string_buffer(4096)
+        out = C.create_string_buffer(4096)
+        N.check(N.lib().e3d_config_read(str(path).encode(), C.byref(p), mesh, 4096, out, 4096))
+        return cls(p, mesh.value.decode(), out.value.decode())
+
+
+class EulerSolver:
+    """One case resident on a HIP device (FlowField + EulerSolver of the reference)."""
+
+    def __init__(self, mesh: Mesh, params: N.Params, device: int = 0):
+        h = C.c_void_p()
+        N.check(N.lib().e3d_create(mesh._h, C.byref(params), device, C.byref(h)))
+        self._h = h
+        info = mesh.info()
+        self.n_elem = info["n_elem"]
+        self.n_face = info["n_face"]
+        self.n_ghost = info["n_ghost"]
+        self.n_tot = self.n_elem + self.n_ghost
+        self.params = params
+
+    def set_state(self, state: dict) -> None:
+        arrs = [np.ascontiguousarray(state[k], dtype=np.float64) for k in STATE_FIELDS]
+        for a in arrs:
+            if a.shape != (self.n_tot,):
+                raise ValueError(f"state arrays must have {self.n_tot} values")
+        ptrs = (N.DP * 8)(*[N.dptr(a) for a in arrs])
+        N.check(N.lib().e3d_set_state(self._h, ptrs))
+
+    def get_state(self) -> dict:
+        out = {k: np.empty(self.n_tot) for k in STATE_FIELDS}
+        ptrs = (N.DP * 8)(*[N.dptr(out[k]) for k in STATE_FIELDS])
+        N.check(N.lib().e3d_get_state(self._h, ptrs))
+        return out
+
+    def update_bc(self) -> None:
+        N.check(N.lib().e3d_update_bc(self._h))
+
+    def face_flux(self) -> tuple[np.ndarray, np.ndarray]:
+        F = np.empty(5 * self.n_face)
+        FS = np.empty(5 * self.n_face)
+        N.check(N.lib().e3d_face_flux(self._h, N.dptr(F), N.dptr(FS)))
+        return F, FS
+
+    def residual(self) -> np.ndarray:
+        R = np.empty(5 * self.n_elem)
+        N.check(N.lib().e3d_residual(self._h, N.dptr(R)))
+        return R
+
+    def timestep(self) -> np.ndarray:
+        dt = np.empty(self.n_elem)
+        N.check(N.lib().e3d_timestep(self._h, N.dptr(dt)))
+        return dt
+
+    def iterate(self, n: int) -> np.ndarray:
+        sums = np.empty(max(n, 1))
+        N.check(N.lib().e3d_iterate(self._h, int(n), N.dptr(sums)))
+        return sums[:n]
+
+    def aero(self) -> tuple[np.ndarray, np.ndarray]:
+        cf = np.empty(3)
+        cm = np.empty(3)
+        N.check(N.lib().e3d_aero(self._h, N.dptr(cf), N.dptr(cm)))
+        return cf, cm
+
+    def run(self, max_rows: int | None = None) -> dict:
+        rows_n = max_rows if max_rows is not None else self.params.max_iter + 1
+        rows = np.zeros(8 * max(rows_n, 1))
+        nrows = C.c_int()
+        iters = C.c_int()
+        coeffs = np.empty(3)
+        cf = np.empty(3)
+        cm = np.empty(3)
+        N.check(N.lib().e3d_run(self._h, N.dptr(rows), rows_n, C.byref(nrows), C.byref(iters), N.dptr(coeffs),
+                                N.dptr(cf), N.dptr(cm)))
+        k = min(nrows.value, rows_n)
+        return {"rows": rows[: 8 * k].reshape(k, 8), "iterations": iters.value, "CL": coeffs[0], "CD": coeffs[1],
+                "CM": coeffs[2], "force_coeff": cf, "moment_coeff": cm}
+
+    def elapsed_ms(self) -> float:
+        ms = C.c_double()
+        N.check(N.lib().e3d_last_elapsed_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    def set_kernel_timing(self, enable: bool) -> None:
+        N.check(N.lib().e3d_set_kernel_timing(self._h, int(enable)))
+
+    def kernel_timing(self) -> tuple[float, int, str]:
+        """(mean ms per launch, launches, kernel name) of the stage kernel since timing was enabled."""
+        ms = C.c_double()
+        n = C.c_int()
+        name = C.create_string_buffer(128)
+        N.check(N.lib().e3d_kernel_timing(self._h, C.byref(ms), C.byref(n), name, 128))
+        return ms.value, n.value, name.value.decode()
+
+    def close(self) -> None:
+        if self._h:
+            N.lib().e3d_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,150 @@
+/*
+ * e3d.h — C ABI of the MI355X-native Euler3D explicit-solver hot path (libe3d_hip.so).
+ *
+ * Drop-in boundary for the reference's in-process solver seam. Each entry point names the
+ * reference interface it replaces (paths are in the Euler3D source tree):
+ *
+ *   EulerSolver(FlowField&, MPIHandler&, Mesh<MeshPartition>&, SimConfig&, Metrics&)
+ *       include/solver/EulerSolver.hpp:19-23          -> e3d_create
+ *   EulerSolver::Run()            src/solver/EulerSolver.cpp:60-167   -> e3d_run / e3d_iterate
+ *   EulerSolver::updateBC()       src/solver/EulerSolver.cpp:201-251  -> e3d_update_bc
+ *   Solver::Roe / Solver::Fc      src/solver/ConvectiveFluxesSchemes.cpp:10-187 -> e3d_face_flux
+ *   EulerSolver::computeResidual  src/solver/EulerSolver.cpp:303-359  -> e3d_residual
+ *   TimeStepMethod2               src/solver/TimeIntegration.cpp:8-40 -> e3d_timestep
+ *   AeroCoefficients::Update      src/solver/AeroCoefficients.cpp:38-52 -> e3d_aero
+ *   FlowField set/get             include/solver/FlowField.hpp:29-115 -> e3d_set_state / e3d_get_state
+ *   SimConfig(file)               src/parser/SimConfig.cpp:15-236      -> e3d_config_read
+ *   SU2MeshParser / MeshPartition src/parser/SU2MeshParser.cpp:12-240,
+ *                                 src/parser/MeshPartition.cpp:15-112  -> e3d_mesh_read
+ *   SU2Writer                     src/partition/SU2Writer.cpp:17-123   -> e3d_mesh_write
+ *   Mesh::solveConnectivity + Metrics  include/mesh/Mesh.hpp:36-105,
+ *                                 src/mesh/Metrics.cpp:9-56            -> e3d_mesh_preprocess
+ *
+ * Conventions. Every function returns 0 (E3D_OK) or a positive error code; the message of the
+ * last failure on the calling thread is e3d_last_error(). Host arrays are caller-owned and copied
+ * in or out; device memory is owned by the opaque context. One host thread per context; contexts
+ * are independent, several may share a device. Cell-indexed arrays follow the reference's
+ * FlowField numbering: interior cells 0..n_elem-1, then ghost cells n_elem..n_elem+n_ghost-1
+ * (ghost k sits on boundary face n_face_int + k). Per-face and per-cell 5-vectors are
+ * component-minor: out[5*i + k], k = (rho, rho u, rho v, rho w, rho H) as in ResidualVar.
+ * All arithmetic is IEEE FP64.
+ */
+#ifndef E3D_H
+#define E3D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define E3D_OK 0
+#define E3D_ERR_ARG 1
+#define E3D_ERR_IO 2
+#define E3D_ERR_MESH 3
+#define E3D_ERR_DEVICE 4
+#define E3D_ERR_UNSUPPORTED 5
+
+#define E3D_SCHEME_ROE 0
+#define E3D_SCHEME_AUSM 1
+
+typedef struct e3d_mesh e3d_mesh;
+typedef struct e3d_ctx e3d_ctx;
+
+/* Solver parameters, the fields of SimConfig that the hot path reads
+ * (include/parser/SimConfig.hpp; keys parsed in src/parser/SimConfig.cpp:49-175). */
+typedef struct e3d_params {
+	double mach;              /* SPEED_VALUE (SPEED_OPTION 0) or derived from m/s (SimConfig.cpp:176-182) */
+	double aoa_deg;           /* AOA */
+	double gamma;             /* GAMMA */
+	double cfl;               /* CFL */
+	double sref, cref;        /* SREF, CREF */
+	double ref_point[3];      /* MESH_REF_POINT_X/Y/Z */
+	int orient_cl[2];         /* (axis, sign) of MESH_ORIENTATION_CL (SimConfig.cpp:185-207) */
+	int orient_cd[2];         /* (axis, sign) of MESH_ORIENTATION_CD */
+	int orient_cm[2];         /* derived, SimConfig.cpp:231 */
+	int scheme;               /* SCHEME: 0 Roe, 1 AUSM */
+	int time_integration;     /* TIME_INTEGRATION: only 2 (RK5) is on this path */
+	int residual_smoothing;   /* RESIDUAL_SMOOTHING: must be 0 on this path */
+	int max_iter;             /* MAX_ITER */
+	int sampling;             /* SAMPLING */
+	double min_residual;      /* MIN_RESIDUAL */
+	double min_aerocoeff_error; /* MIN_AEROCOEFF_ERROR */
+} e3d_params;
+
+typedef struct e3d_mesh_info {
+	int n_node, n_elem, n_face, n_face_int, n_ghost, n_wall, n_far, n_sym;
+} e3d_mesh_info;
+
+const char *e3d_last_error(void);
+const char *e3d_version(void);
+
+/* ---- configuration (.e3d) ---------------------------------------------------------------- */
+int e3d_params_default(e3d_params *out);
+/* Parses an .e3d file. mesh_path receives the first partition file (PARTITION_FILES pattern with
+ * '#' -> 0), out_dir the directory of OUTPUT_FILE. Partition counts other than 1 are rejected. */
+int e3d_config_read(const char *path, e3d_params *out, char *mesh_path, int mesh_path_len, char *out_dir,
+                    int out_dir_len);
+
+/* ---- meshes ------------------------------------------------------------------------------- */
+/* SU2 (.su2) or partition (.par, require_par = 1: the NINTERNAL_MARKER block must be present) */
+int e3d_mesh_read(const char *path, int require_par, e3d_mesh **out);
+/* SU2Writer layout; precision = digits after the point of the %e coordinates; par_block adds
+ * "NINTERNAL_MARKER= 0" so the file is a one-partition .par */
+int e3d_mesh_write(const e3d_mesh *m, const char *path, int precision, int par_block);
+/* Synthetic meshes. tags6: six comma-separated tags for the x-, x+, y-, y+, z-, z+ sides. */
+int e3d_mesh_gen_box(int nx, int ny, int nz, double lx, double ly, double lz, const char *tags6, e3d_mesh **out);
+int e3d_mesh_gen_naca(int ni, int nj, int nk, double thickness, double camber, double camber_pos,
+                      double wall_spacing, double farfield_radius, double dz, e3d_mesh **out);
+int e3d_mesh_gen_bump(int nx, int ny, int nz, double jitter, unsigned long long seed, int permute, e3d_mesh **out);
+int e3d_mesh_gen_wing(int ni, int nj, int nk, double thickness, double sweep_deg, double taper, double semispan,
+                      double farfield, e3d_mesh **out);
+/* Connectivity, ghost cells and FP64 metrics, numbered as the reference numbers them. */
+int e3d_mesh_preprocess(e3d_mesh *m, double cref);
+int e3d_mesh_get_info(const e3d_mesh *m, e3d_mesh_info *out);
+/* Copies a named array out (capacity in elements; *n receives the full length). Names:
+ * int32: face2element, face2nodeStart, face2node, element2faceStart, element2face,
+ *        element2element, wallGhost, wallCell, wallFace, farGhost, farCell, farFace, symGhost,
+ *        symCell, symFace, ghostKind, elemVtk, elemStart, elemNodes
+ * f64:   faceNormals, faceNormalsUnit, faceSurfaces, faceCenters, cellVolumes, cellCentroids, xyz */
+int e3d_mesh_get_i32(const e3d_mesh *m, const char *name, int *out, long long capacity, long long *n);
+int e3d_mesh_get_f64(const e3d_mesh *m, const char *name, double *out, long long capacity, long long *n);
+void e3d_mesh_free(e3d_mesh *m);
+
+/* ---- device solver context ---------------------------------------------------------------- */
+/* Uploads a preprocessed mesh to HIP device `device` and initialises the freestream state
+ * (FlowField::Initialize, src/solver/FlowField.cpp:68-101). */
+int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out);
+/* fields = {rho, u, v, w, p, E, H, M}, each n_elem + n_ghost values */
+int e3d_set_state(e3d_ctx *c, const double *const fields[8]);
+int e3d_get_state(e3d_ctx *c, double *const fields[8]);
+int e3d_update_bc(e3d_ctx *c);
+/* per-face flux of the current state as computeResidual dispatches it; F unscaled, FS = F*S
+ * (either may be NULL), 5*n_face values each */
+int e3d_face_flux(e3d_ctx *c, double *F, double *FS);
+/* residual of the current state (no BC update), 5*n_elem values */
+int e3d_residual(e3d_ctx *c, double *R);
+int e3d_timestep(e3d_ctx *c, double *dt);
+/* n iterations of Run's body: updateBC, computeResidual, RungeKutta (5 stages). rms_sum (n values
+ * or NULL) receives sum over interior cells of R_rho^2 of each iteration's last stage
+ * (EulerSolver.cpp:96); RMS = sqrt(rms_sum / n_elem). */
+int e3d_iterate(e3d_ctx *c, int n, double *rms_sum);
+/* force and moment coefficients of the current state (AeroCoefficients::Update) */
+int e3d_aero(e3d_ctx *c, double cf[3], double cm[3]);
+/* Run loop with MIN_RESIDUAL, SAMPLING and coefficient convergence (EulerSolver.cpp:60-167).
+ * rows: up to max_rows rows of 8 doubles (iter, rms, Cx, Cy, Cz, CMx, CMy, CMz) as residuals.dat
+ * holds them. coeffs = CL, CD, CM after the loop. */
+int e3d_run(e3d_ctx *c, double *rows, int max_rows, int *n_rows, int *iters, double coeffs[3], double cf[3],
+            double cm[3]);
+/* device time of the last e3d_iterate / e3d_run call in milliseconds (HIP events) */
+int e3d_last_elapsed_ms(e3d_ctx *c, double *ms);
+/* Per-launch timing of the dominant (per-stage residual + update) kernel: when enabled, every
+ * launch of it during e3d_iterate / e3d_run is bracketed by HIP events on the context's stream.
+ * e3d_kernel_timing reports the mean duration of those launches, their count and the kernel's
+ * name, then clears the record. No counterpart in the reference (measurement hook). */
+int e3d_set_kernel_timing(e3d_ctx *c, int enable);
+int e3d_kernel_timing(e3d_ctx *c, double *avg_ms, int *launches, char *name, int name_len);
+void e3d_destroy(e3d_ctx *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

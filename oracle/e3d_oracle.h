@@ -1,0 +1,91 @@
+/*
+ * CPU ORACLE of the Euler3D explicit-solver hot path. TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, single-threaded restatement of the reference's FP64 arithmetic, in the reference's
+ * evaluation order, used to check the HIP path (tests/, the smoke check and the cpu_baseline leg
+ * of the benchmark). The product never links, loads or calls it.
+ *
+ * Parity status of each function (what pins it):
+ *   or_freestream      FlowField.cpp:68-101        golden fixtures (reference harness "mesh"/"run")
+ *   or_update_bc       EulerSolver.cpp:201-251,
+ *                      PhysicalBC.cpp:11-231       golden fixtures ("flux": bc_* states)
+ *   or_roe             ConvectiveFluxesSchemes.cpp:10-165   golden fixtures (per-face flux)
+ *   or_fc              ConvectiveFluxesSchemes.cpp:167-187  golden fixtures (per-face flux)
+ *   or_face_flux       EulerSolver.cpp:306-358     golden fixtures (per-face flux)
+ *   or_residual        EulerSolver.cpp:303-359     golden fixtures (residual)
+ *   or_timestep        TimeIntegration.cpp:8-40    golden fixtures (dt)
+ *   or_rk5             EulerSolver.cpp:425-472, FlowField.cpp:140-171  golden fixtures (rk_*)
+ *   or_rms_sum         EulerSolver.cpp:382-389     golden fixtures (rk_rms_sum, run_rms_sum)
+ *   or_aero            AeroCoefficients.cpp:5-52   golden fixtures (force/moment coeff, CL/CD/CM)
+ *   or_run             EulerSolver.cpp:60-167      golden fixtures (residuals.dat, final state)
+ *   or_ausm            no reference implementation exists (ConvectiveFluxesSchemes.h:27-30 only
+ *                      declares it): PARITY UNPINNED. Restates Liou & Steffen, J. Comput. Phys.
+ *                      107 (1993) 23-39, and is checked by invariants only (consistency
+ *                      F(U,U,n) = physical flux, supersonic upwinding, conservation).
+ */
+#ifndef E3D_ORACLE_H
+#define E3D_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+	int nElem, nFace, nFaceInt, nGhost;
+	const int *face2element;      /* 2*nFace, (elem0, elem1) */
+	const int *element2faceStart; /* nElem+1 */
+	const int *element2face;      /* local-face order */
+	const double *faceNormal;     /* 3*nFace area-weighted */
+	const double *faceUnitNormal; /* 3*nFace */
+	const double *faceArea;       /* nFace */
+	const double *faceCenter;     /* 3*nFace */
+	const double *cellVolume;     /* nElem */
+	int nWall, nFar, nSym;
+	const int *wallGhost, *wallCell, *wallFace;
+	const int *farGhost, *farCell, *farFace;
+	const int *symGhost, *symCell, *symFace;
+} or_mesh;
+
+typedef struct {
+	double gamma, cfl, mach, aoa_deg;
+	double sref, cref, ref_point[3];
+	int orient[3][2]; /* CL, CD, CM: (axis, sign) */
+	int scheme;       /* 0 Roe, 1 AUSM */
+	int max_iter, sampling;
+	double min_residual, min_aerocoeff_error;
+	/* filled by or_freestream */
+	double rho_inf, p_inf, u_inf, v_inf, w_inf, E_inf, H_inf, M_inf;
+} or_params;
+
+typedef struct {
+	double *rho, *u, *v, *w, *p, *E, *H, *M; /* nElem + nGhost */
+} or_state;
+
+void or_freestream(or_params *prm);
+void or_init_state(const or_mesh *m, const or_params *prm, or_state *s);
+void or_update_bc(const or_mesh *m, const or_params *prm, or_state *s);
+void or_roe(const or_mesh *m, const or_params *prm, const or_state *s, int face, double out[5]);
+void or_ausm(const or_mesh *m, const or_params *prm, const or_state *s, int face, double out[5]);
+void or_fc(const or_mesh *m, const or_state *s, int ghost, int face, double V, double out[5]);
+/* per-face flux as the residual dispatches it; F unscaled and FS = F*S (5 per face), kind per face
+ * (0 interior, 1 wall, 2 symmetry, 3 farfield) */
+void or_face_flux(const or_mesh *m, const or_params *prm, const or_state *s, double *F, double *FS, int *kind);
+void or_residual(const or_mesh *m, const or_params *prm, const or_state *s, double *R /* 5*nElem */);
+void or_timestep(const or_mesh *m, const or_params *prm, const or_state *s, double *dt);
+/* One RK5 step; R holds the residual of the current state on entry (as Run computes it before
+ * calling the integrator) and the last-stage residual on exit. */
+void or_rk5(const or_mesh *m, const or_params *prm, or_state *s, double *R);
+double or_rms_sum(const double *R, int nElem);
+/* wall-face force and moment coefficients (AeroCoefficients::Update) */
+void or_aero(const or_mesh *m, const or_params *prm, const or_state *s, double cf[3], double cm[3]);
+/* Run loop. Writes per-row history (up to max_rows rows of 8 doubles: iter, rms, Cx, Cy, Cz,
+ * CMx, CMy, CMz as residuals.dat holds them) and returns the number of rows; *iters receives the
+ * final iteration count and coeffs[3] CL, CD, CM. R (5*nElem) is scratch and holds the last
+ * residual on exit. */
+int or_run(const or_mesh *m, const or_params *prm, or_state *s, double *R, double *rows, int max_rows,
+           int *iters, double coeffs[3], double cf[3], double cm[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
