@@ -3,11 +3,13 @@
 #include "../../include/e3d.h"
 #include "device.hpp"
 #include "e3d_host.hpp"
+#include "plan.hpp"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -86,12 +88,21 @@ struct e3d_ctx {
 	bool timing = false;
 	std::vector<hipEvent_t> evPool;
 	size_t evUsed = 0;
-	// mesh
+	bool fused = true;       // fused stage kernel (off: the three-pass kBc/kFlux/kStage form)
+	std::vector<int> new2old; // device cell id -> reference cell id (interior)
+	// mesh, device numbering
 	DBuf<int> e0, e1, fstart, flist;
 	DBuf<double> nx, ny, nz, S, vol, sx, sy, sz;
 	DBuf<int8_t> gkind;
-	// state
-	DBuf<double> st[8]; // rho u v w p E H M
+	// fused-stage plan
+	e3d::DevPlan plan{};
+	DBuf<int> blockSlot, se0, se1;
+	DBuf<int8_t> skind;
+	DBuf<double> snx, sny, snz, sS;
+	DBuf<uint16_t> csr;
+	// state: two buffers, st[cur] is the current one
+	DBuf<double> st[2][8]; // rho u v w p E H M
+	int cur = 0;
 	DBuf<double> W0, fs, partial, scalars;
 	// aero (wall list order)
 	int nWall = 0;
@@ -99,9 +110,11 @@ struct e3d_ctx {
 	DBuf<double> wN[3], wC[3];
 	double cpFactor = 0.0;
 
-	e3d::DevState state() {
-		return {st[0].p, st[1].p, st[2].p, st[3].p, st[4].p, st[5].p, st[6].p, st[7].p};
+	e3d::DevState state(int which) {
+		auto &b = st[which];
+		return {b[0].p, b[1].p, b[2].p, b[3].p, b[4].p, b[5].p, b[6].p, b[7].p};
 	}
+	e3d::DevState state() { return state(cur); }
 	void sync() { hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
 	void check() { hipCheck(hipGetLastError(), "kernel launch"); }
 };
@@ -127,9 +140,10 @@ void fillFree(const e3d_params &p, e3d::DevFree &q) {
 void initFreestream(e3d_ctx *c) {
 	std::vector<double> h(c->nTot);
 	const double vals[8] = {c->fr.rho_inf, c->fr.u_inf, c->fr.v_inf, c->fr.w_inf, c->fr.p_inf, c->fr.E_inf, c->fr.H_inf, c->fr.M_inf};
+	c->cur = 0;
 	for (int k = 0; k < 8; k++) {
 		std::fill(h.begin(), h.end(), vals[k]);
-		c->st[k].upload(h.data(), c->nTot, c->stream);
+		c->st[0][k].upload(h.data(), c->nTot, c->stream);
 	}
 	c->sync();
 	c->machDerived = 0;
@@ -139,6 +153,9 @@ void runStage(e3d_ctx *c, int stage, double *rmsPartial) {
 	static const double alphas[5] = {0.0533, 0.1263, 0.2375, 0.4414, 1.0};
 	auto s = c->state();
 	hipEvent_t evA = nullptr, evB = nullptr;
+	auto record = [&](hipEvent_t e) {
+		if (e) hipCheck(hipEventRecord(e, c->stream), "hipEventRecord");
+	};
 	if (c->timing) {
 		if (c->evUsed + 2 > c->evPool.size()) {
 			for (int i = 0; i < 64; i++) {
@@ -149,16 +166,36 @@ void runStage(e3d_ctx *c, int stage, double *rmsPartial) {
 		}
 		evA = c->evPool[c->evUsed++];
 		evB = c->evPool[c->evUsed++];
-		hipCheck(hipEventRecord(evA, c->stream), "hipEventRecord");
 	}
-	e3d::launchBc(c->dm, s, c->fr, c->machDerived, c->stream);
-	e3d::launchFlux(c->dm, s, c->fr, c->prm.scheme, c->fs.p, nullptr, c->stream);
-	e3d::launchStage(c->dm, s, c->fr, c->fs.p, c->W0.p, alphas[stage], stage == 0, rmsPartial, nullptr, c->stream);
-	if (c->timing) hipCheck(hipEventRecord(evB, c->stream), "hipEventRecord");
+	if (c->fused) {
+		auto o = c->state(c->cur ^ 1);
+		record(evA);
+		e3d::launchStageFused(c->dm, c->plan, s, o, s.M, c->fr, c->prm.scheme, c->W0.p, alphas[stage], stage == 0,
+		                      c->machDerived, rmsPartial, c->stream);
+		record(evB);
+		c->cur ^= 1;
+	} else {
+		record(evA);
+		e3d::launchBc(c->dm, s, c->fr, c->machDerived, c->stream);
+		e3d::launchFlux(c->dm, s, c->fr, c->prm.scheme, c->fs.p, nullptr, c->stream);
+		e3d::launchStage(c->dm, s, c->fr, c->fs.p, c->W0.p, alphas[stage], stage == 0, rmsPartial, nullptr, c->stream);
+		record(evB);
+	}
 	c->machDerived = 1;
 }
 
-const char *kStageKernelName = "kBc+kFlux+kStage";
+const char *stageKernelName(const e3d_ctx *c) { return c->fused ? "kStageFused" : "kBc+kFlux+kStage"; }
+
+// host reference-order arrays <-> device-order arrays (interior permuted, ghosts as they are)
+void toDevice(const e3d_ctx *c, const double *ref, std::vector<double> &dev) {
+	dev.resize(c->nTot);
+	for (int i = 0; i < c->nElem; i++) dev[i] = ref[c->new2old[i]];
+	for (int i = c->nElem; i < c->nTot; i++) dev[i] = ref[i];
+}
+void toReference(const e3d_ctx *c, const std::vector<double> &dev, double *ref) {
+	for (int i = 0; i < c->nElem; i++) ref[c->new2old[i]] = dev[i];
+	for (int i = c->nElem; i < c->nTot; i++) ref[i] = dev[i];
+}
 
 // one iteration of Run's body; the sum of R_rho^2 of the last stage lands in *dsum (device)
 void runIteration(e3d_ctx *c, double *dsum) {
@@ -413,84 +450,58 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		c->nTot = sm.nElem + sm.nGhost;
 		const size_t nF = sm.nFace, nE = sm.nElem;
 		hipStream_t s = c->stream;
+		const char *orderEnv = std::getenv("E3D_CELL_ORDER");
+		const char *pathEnv = std::getenv("E3D_STAGE_PATH");
+		int order = (orderEnv && std::string(orderEnv) == "reference") ? e3d::ORDER_REFERENCE : e3d::ORDER_MORTON;
+		c->fused = !(pathEnv && std::string(pathEnv) == "unfused");
+		e3d::HostPlan P = e3d::buildPlan(sm, order, e3d::kCellsPerBlock);
+		if ((size_t) P.maxSlots * 5 * sizeof(double) > 160 * 1024)
+			return fail(E3D_ERR_UNSUPPORTED, "a block of the fused stage needs more LDS than a CU has");
+		c->new2old = P.new2old;
 
-		// faces: SoA
-		std::vector<int> he0(nF), he1(nF);
-		std::vector<double> hx(nF), hy(nF), hz(nF);
-		for (size_t f = 0; f < nF; f++) {
-			he0[f] = sm.face2element[2 * f];
-			he1[f] = sm.face2element[2 * f + 1];
-			hx[f] = sm.faceUnitNormal[3 * f];
-			hy[f] = sm.faceUnitNormal[3 * f + 1];
-			hz[f] = sm.faceUnitNormal[3 * f + 2];
-		}
-		c->e0.alloc(nF), c->e0.upload(he0.data(), nF, s);
-		c->e1.alloc(nF), c->e1.upload(he1.data(), nF, s);
-		c->nx.alloc(nF), c->nx.upload(hx.data(), nF, s);
-		c->ny.alloc(nF), c->ny.upload(hy.data(), nF, s);
-		c->nz.alloc(nF), c->nz.upload(hz.data(), nF, s);
-		c->S.alloc(nF), c->S.upload(sm.faceArea.data(), nF, s);
-		c->sync();
-
-		// ghosts
-		std::vector<int8_t> gk(sm.nGhost);
-		for (int k = 0; k < sm.nGhost; k++) gk[k] = (int8_t) (sm.ghostKind[k] == e3d::BND_WALL ? 1 : sm.ghostKind[k] == e3d::BND_SYM ? 2 : 0);
-		c->gkind.alloc(sm.nGhost), c->gkind.upload(gk.data(), sm.nGhost, s);
-
-		// cells: static time-step sums in local-face order, face CSR in ascending face id
-		std::vector<double> hsx(nE), hsy(nE), hsz(nE);
-		std::vector<int> hstart(nE + 1), hlist(sm.element2face.size());
-		for (size_t e = 0; e < nE; e++) {
-			double ax = 0, ay = 0, az = 0;
-			int b = sm.element2faceStart[e], en = sm.element2faceStart[e + 1];
-			for (int j = b; j < en; j++) {
-				int f = sm.element2face[j];
-				ax += std::fabs(sm.faceNormal[3 * (size_t) f]);
-				ay += std::fabs(sm.faceNormal[3 * (size_t) f + 1]);
-				az += std::fabs(sm.faceNormal[3 * (size_t) f + 2]);
-			}
-			hsx[e] = ax, hsy[e] = ay, hsz[e] = az;
-			hstart[e] = b;
-			std::vector<int> fl(sm.element2face.begin() + b, sm.element2face.begin() + en);
-			std::sort(fl.begin(), fl.end());
-			for (int j = 0; j < en - b; j++) {
-				int f = fl[j];
-				bool second = sm.face2element[2 * (size_t) f + 1] == (int) e && sm.face2element[2 * (size_t) f] != (int) e;
-				hlist[b + j] = second ? (int) ((unsigned) f | 0x80000000u) : f;
-			}
-		}
-		hstart[nE] = (int) sm.element2face.size();
-		c->vol.alloc(nE), c->vol.upload(sm.cellVolume.data(), nE, s);
-		c->sx.alloc(nE), c->sx.upload(hsx.data(), nE, s);
-		c->sy.alloc(nE), c->sy.upload(hsy.data(), nE, s);
-		c->sz.alloc(nE), c->sz.upload(hsz.data(), nE, s);
-		c->fstart.alloc(nE + 1), c->fstart.upload(hstart.data(), nE + 1, s);
-		c->flist.alloc(hlist.size()), c->flist.upload(hlist.data(), hlist.size(), s);
-		c->sync();
-
+		// faces in reference order, device cell ids
+		c->e0.alloc(nF), c->e0.upload(P.fe0.data(), nF, s);
+		c->e1.alloc(nF), c->e1.upload(P.fe1.data(), nF, s);
+		c->nx.alloc(nF), c->nx.upload(P.fnx.data(), nF, s);
+		c->ny.alloc(nF), c->ny.upload(P.fny.data(), nF, s);
+		c->nz.alloc(nF), c->nz.upload(P.fnz.data(), nF, s);
+		c->S.alloc(nF), c->S.upload(P.fS.data(), nF, s);
+		c->gkind.alloc(sm.nGhost), c->gkind.upload(P.gkind.data(), sm.nGhost, s);
+		// cells in device order
+		c->vol.alloc(nE), c->vol.upload(P.vol.data(), nE, s);
+		c->sx.alloc(nE), c->sx.upload(P.sx.data(), nE, s);
+		c->sy.alloc(nE), c->sy.upload(P.sy.data(), nE, s);
+		c->sz.alloc(nE), c->sz.upload(P.sz.data(), nE, s);
+		c->fstart.alloc(nE + 1), c->fstart.upload(P.fstart.data(), nE + 1, s);
+		c->flist.alloc(P.flist.size()), c->flist.upload(P.flist.data(), P.flist.size(), s);
+		// fused-stage plan
+		size_t nS = P.se0.size();
+		c->blockSlot.alloc(P.blockSlot.size()), c->blockSlot.upload(P.blockSlot.data(), P.blockSlot.size(), s);
+		c->se0.alloc(nS), c->se0.upload(P.se0.data(), nS, s);
+		c->se1.alloc(nS), c->se1.upload(P.se1.data(), nS, s);
+		c->skind.alloc(nS), c->skind.upload(P.skind.data(), nS, s);
+		c->snx.alloc(nS), c->snx.upload(P.snx.data(), nS, s);
+		c->sny.alloc(nS), c->sny.upload(P.sny.data(), nS, s);
+		c->snz.alloc(nS), c->snz.upload(P.snz.data(), nS, s);
+		c->sS.alloc(nS), c->sS.upload(P.sS.data(), nS, s);
+		c->csr.alloc(P.csr.size()), c->csr.upload(P.csr.data(), P.csr.size(), s);
+		c->plan = {P.nBlocks, P.maxSlots, P.csrStride, c->blockSlot.p, c->se0.p, c->se1.p, c->skind.p,
+		           c->snx.p, c->sny.p, c->snz.p, c->sS.p, c->csr.p};
 		// wall list in the reference's order for the force integration
-		c->nWall = (int) sm.wallGhost.size();
+		c->nWall = (int) P.wallCell.size();
 		if (c->nWall) {
-			std::vector<double> h[6];
-			for (auto &v : h) v.resize(c->nWall);
-			for (int i = 0; i < c->nWall; i++) {
-				int f = sm.wallFace[i];
-				for (int k = 0; k < 3; k++) {
-					h[k][i] = sm.faceNormal[3 * (size_t) f + k];
-					h[3 + k][i] = sm.faceCenter[3 * (size_t) f + k];
-				}
-			}
-			c->wallCell.alloc(c->nWall), c->wallCell.upload(sm.wallCell.data(), c->nWall, s);
+			c->wallCell.alloc(c->nWall), c->wallCell.upload(P.wallCell.data(), c->nWall, s);
 			for (int k = 0; k < 3; k++) {
-				c->wN[k].alloc(c->nWall), c->wN[k].upload(h[k].data(), c->nWall, s);
-				c->wC[k].alloc(c->nWall), c->wC[k].upload(h[3 + k].data(), c->nWall, s);
+				c->wN[k].alloc(c->nWall), c->wN[k].upload(P.wallN[k].data(), c->nWall, s);
+				c->wC[k].alloc(c->nWall), c->wC[k].upload(P.wallC[k].data(), c->nWall, s);
 			}
-			c->sync();
 		}
+		c->sync();
 		double uinf2 = c->fr.u_inf * c->fr.u_inf + c->fr.v_inf * c->fr.v_inf + c->fr.w_inf * c->fr.w_inf;
 		c->cpFactor = 1 / (0.5 * c->fr.rho_inf * uinf2);
 
-		for (auto &b : c->st) b.alloc(c->nTot);
+		for (auto &buf : c->st)
+			for (auto &b : buf) b.alloc(c->nTot);
 		c->W0.alloc(5 * nE);
 		c->fs.alloc(5 * nF);
 		c->partial.alloc(std::max(1, e3d::stageBlocks(c->nElem)));
@@ -515,8 +526,12 @@ int e3d_set_state(e3d_ctx *c, const double *const fields[8]) {
 		if (!fields[k]) return fail(E3D_ERR_ARG, "null state field");
 	return guarded([&] {
 		hipCheck(hipSetDevice(c->device), "hipSetDevice");
-		for (int k = 0; k < 8; k++) c->st[k].upload(fields[k], c->nTot, c->stream);
-		c->sync();
+		std::vector<double> h;
+		for (int k = 0; k < 8; k++) {
+			toDevice(c, fields[k], h);
+			c->st[c->cur][k].upload(h.data(), c->nTot, c->stream);
+			c->sync();
+		}
 		c->machDerived = 0;
 		return E3D_OK;
 	});
@@ -530,11 +545,15 @@ int e3d_get_state(e3d_ctx *c, double *const fields[8]) {
 			e3d::launchMach(c->dm, c->state(), c->fr, c->stream);
 			c->check();
 		}
+		std::vector<double> h(c->nTot);
 		for (int k = 0; k < 8; k++)
-			if (fields[k])
-				hipCheck(hipMemcpyAsync(fields[k], c->st[k].p, sizeof(double) * c->nTot, hipMemcpyDeviceToHost, c->stream),
+			if (fields[k]) {
+				hipCheck(hipMemcpyAsync(h.data(), c->st[c->cur][k].p, sizeof(double) * c->nTot, hipMemcpyDeviceToHost,
+				                        c->stream),
 				         "hipMemcpyAsync D2H");
-		c->sync();
+				c->sync();
+				toReference(c, h, fields[k]);
+			}
 		return E3D_OK;
 	});
 }
@@ -593,7 +612,10 @@ int e3d_residual(e3d_ctx *c, double *R) {
 		std::vector<double> h(5 * (size_t) c->nElem);
 		hipCheck(hipMemcpyAsync(h.data(), dR.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream), "D2H");
 		c->sync();
-		toComponentMinor(h, c->nElem, R);
+		std::vector<double> dev(5 * (size_t) c->nElem);
+		toComponentMinor(h, c->nElem, dev.data());
+		for (int i = 0; i < c->nElem; i++)
+			for (int k = 0; k < 5; k++) R[5 * (size_t) c->new2old[i] + k] = dev[5 * (size_t) i + k];
 		return E3D_OK;
 	});
 }
@@ -606,8 +628,10 @@ int e3d_timestep(e3d_ctx *c, double *dt) {
 		d.alloc(c->nElem);
 		e3d::launchTimestep(c->dm, c->state(), c->fr, d.p, c->stream);
 		c->check();
-		hipCheck(hipMemcpyAsync(dt, d.p, sizeof(double) * c->nElem, hipMemcpyDeviceToHost, c->stream), "D2H");
+		std::vector<double> h(c->nElem);
+		hipCheck(hipMemcpyAsync(h.data(), d.p, sizeof(double) * c->nElem, hipMemcpyDeviceToHost, c->stream), "D2H");
 		c->sync();
+		for (int i = 0; i < c->nElem; i++) dt[c->new2old[i]] = h[i];
 		return E3D_OK;
 	});
 }
@@ -639,7 +663,7 @@ int e3d_aero(e3d_ctx *c, double cf[3], double cm[3]) {
 		double h[6] = {0, 0, 0, 0, 0, 0};
 		if (c->nWall) {
 			e3d::launchAero(c->nWall, c->wallCell.p, c->wN[0].p, c->wN[1].p, c->wN[2].p, c->wC[0].p, c->wC[1].p, c->wC[2].p,
-			                c->st[4].p, c->cpFactor, c->fr.p_inf, c->prm.cref, c->prm.sref, c->scalars.p, c->stream);
+			                c->st[c->cur][4].p, c->cpFactor, c->fr.p_inf, c->prm.cref, c->prm.sref, c->scalars.p, c->stream);
 			c->check();
 			hipCheck(hipMemcpyAsync(h, c->scalars.p, sizeof(h), hipMemcpyDeviceToHost, c->stream), "D2H");
 			c->sync();
@@ -742,7 +766,7 @@ int e3d_kernel_timing(e3d_ctx *c, double *avg_ms, int *launches, char *name, int
 		if (avg_ms) *avg_ms = n ? total / n : 0.0;
 		if (launches) *launches = n;
 		if (name && name_len > 0) {
-			std::strncpy(name, kStageKernelName, (size_t) name_len - 1);
+			std::strncpy(name, stageKernelName(c), (size_t) name_len - 1);
 			name[name_len - 1] = 0;
 		}
 		c->evUsed = 0;
@@ -754,7 +778,8 @@ void e3d_destroy(e3d_ctx *c) {
 	if (!c) return;
 	(void) hipSetDevice(c->device);
 	if (c->stream) (void) hipStreamSynchronize(c->stream);
-	for (auto &b : c->st) b.free();
+	for (auto &buf : c->st)
+		for (auto &b : buf) b.free();
 	c->W0.free(), c->fs.free(), c->partial.free(), c->scalars.free();
 	if (c->ev0) (void) hipEventDestroy(c->ev0);
 	if (c->ev1) (void) hipEventDestroy(c->ev1);

@@ -1,16 +1,23 @@
 // Device-resident solver data for one case and the launchers of the HIP kernels that advance it.
 //
-// Layout in HBM (all FP64 unless noted, structure-of-arrays so that a wavefront's 64 lanes load
-// 64 consecutive values):
-//   faces  [nFace]   e0, e1 (int32; e1 >= nElem is ghost e1 - nElem), unit normal nx, ny, nz, area
+// Cells are stored in a locality order (a space-filling-curve sort of the centroids) so that a
+// block of consecutive cells is a compact cluster; ghost cells keep the reference's numbering
+// after the interior ones. The reference numbering survives where it decides bits: every face
+// keeps its reference orientation (elem0 -> elem1) and every cell sums its faces in ascending
+// reference face id.
+//
+// Layout in HBM (FP64 unless noted; structure-of-arrays, so 64 lanes load 64 consecutive values):
+//   faces  [nFace]   e0, e1 (int32, device cell ids; e1 >= nElem is a ghost), unit normal nx, ny,
+//                    nz, area S (reference face order; used by the per-face parity entry points)
 //   ghosts [nGhost]  kind (int8: 0 farfield, 1 wall, 2 symmetry); ghost k sits on face nFaceInt + k
-//   cells  [nElem]   volume, sum|Nx|, sum|Ny|, sum|Nz| (static parts of the local time step),
-//                    face CSR (int32 face id | sign bit, ascending face id: the reference's
-//                    accumulation order)
-//   state  [nElem + nGhost]  rho, u, v, w, p, E, H, M  (ghost part persists between stages, as the
-//                    farfield classifier reads the ghost's previous state)
+//   cells  [nElem]   volume, sum|Nx|, sum|Ny|, sum|Nz| (static parts of the local time step)
+//   state  2 x [nElem + nGhost] rho, u, v, w, p, E, H, M: ping-pong buffers, a stage reads one and
+//                    writes the other (the ghost part carries the farfield ghosts' previous state)
 //   W0     [5][nElem] conservative state at the start of the iteration
-//   flux   [5][nFace] F * S of every face for the current stage
+//   plan   per block of kCellsPerBlock cells: its faces (every face touching the block, faces cut
+//                    by the block boundary appear in both blocks) as slot records e0, e1, kind,
+//                    n, S; per cell the block-local slots of its faces in ascending reference
+//                    face id, sign in bit 15
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,6 +25,8 @@
 #include <cstdint>
 
 namespace e3d {
+
+constexpr int kCellsPerBlock = 256;
 
 struct DevFaces {
 	const int *e0, *e1;
@@ -46,18 +55,35 @@ struct DevMesh {
 	const int8_t *gkind; // nGhost
 };
 
-// per-face flux output: F*S into fs[5][nFace]; when raw != nullptr also the unscaled F
+// block plan of the fused stage kernel
+struct DevPlan {
+	int nBlocks;
+	int maxSlots;            // largest slot count of a block (LDS size)
+	int csrStride;           // slots per cell in the CSR (4 for tets, 6 with hexes), 0xFFFF = none
+	const int *blockSlot;    // nBlocks + 1: slot range of each block
+	const int *e0, *e1;      // per slot: device cell ids (e1 >= nElem: ghost)
+	const int8_t *kind;      // per slot: 0 interior, 1 wall, 2 symmetry, 3 farfield
+	const double *nx, *ny, *nz, *S;
+	const uint16_t *csr;     // nElem * csrStride
+};
+
 void launchBc(const DevMesh &m, DevState s, const DevFree &q, int machDerived, hipStream_t st);
+// per-face flux output: F*S into fs[5][nFace]; when raw != nullptr also the unscaled F
 void launchFlux(const DevMesh &m, DevState s, const DevFree &q, int scheme, double *fs, double *raw, hipStream_t st);
 // gather-only: R[5][nElem] from fs (parity hook)
 void launchGather(const DevMesh &m, const double *fs, double *R, hipStream_t st);
 void launchTimestep(const DevMesh &m, DevState s, const DevFree &q, double *dt, hipStream_t st);
-// one RK stage: gather residual, local dt, W_k = W0 - alpha*dt*R/V, primitive update.
-// stage 0 also stores W0 from the current primitives; when rmsPartial != nullptr each block writes
-// its partial sum of R_rho^2; when Rout != nullptr the residual is stored.
+// Unfused RK stage (needs launchBc + launchFlux before it): gather residual, local dt,
+// W_k = W0 - alpha*dt*R/V, primitive update in place. firstStage also stores W0; when
+// rmsPartial != nullptr each block writes its partial sum of R_rho^2.
 void launchStage(const DevMesh &m, DevState s, const DevFree &q, const double *fs, double *W0, double alpha,
                  int firstStage, double *rmsPartial, double *Rout, hipStream_t st);
 int stageBlocks(int nElem);
+// Fused RK stage: boundary ghosts, face fluxes (once per block, in LDS), residual gather, local
+// dt and update, reading `in` and writing `out` (interior and ghosts).
+void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const double *Mint,
+                      const DevFree &q, int scheme, double *W0, double alpha, int firstStage, int machDerived,
+                      double *rmsPartial, hipStream_t st);
 // deterministic sum of n doubles into out[0] (single workgroup, fixed order)
 void launchSum(const double *in, int n, double *out, hipStream_t st);
 // interior Mach number from the primitives (FlowField.cpp:157), for state read-back

@@ -2,10 +2,18 @@
 //
 // Arithmetic follows the reference expression by expression (file:line given per function) and
 // the file is compiled with -ffp-contract=off, so every add and multiply rounds as the
-// reference's does; FP64 division and sqrt are the IEEE correctly rounded sequences.
-// Assembly is a per-cell gather over a face CSR in ascending face id, the reference's
-// accumulation order, so the result is deterministic and bit-identical to the sequential face
-// loop without atomics.
+// reference's does; FP64 division and sqrt are the IEEE correctly rounded sequences, and x / 2
+// is written x * 0.5 (the same correctly rounded value). Assembly is a per-cell gather in
+// ascending reference face id, the reference's accumulation order, so the result is
+// deterministic and bit-identical to the sequential face loop without atomics.
+//
+// Two forms of an RK stage exist:
+//   kStageFused  the production path: one workgroup per block of kCellsPerBlock consecutive cells
+//                computes every face flux touching the block once into LDS (boundary ghosts
+//                included), then each thread gathers its cell's fluxes from LDS, forms the local
+//                time step and updates the cell, ping-ponging between two state buffers.
+//   kBc, kFlux, kStage (+ kGather, kTimestep)  one pass per step through HBM, used by the
+//                per-face parity entry points (update_bc, face_flux, residual, timestep).
 #include "device.hpp"
 
 namespace e3d {
@@ -13,13 +21,29 @@ namespace e3d {
 namespace {
 
 constexpr int kBlock = 256;
+static_assert(kBlock == kCellsPerBlock, "one thread per cell in the fused stage");
 
 struct Prim {
 	double rho, u, v, w, p, H;
 };
 
+struct Ghost {
+	double rho, u, v, w, p, E, H, M;
+};
+
 __device__ __forceinline__ Prim loadPrim(const DevState &s, int i) {
 	return {s.rho[i], s.u[i], s.v[i], s.w[i], s.p[i], s.H[i]};
+}
+
+__device__ __forceinline__ void storeGhost(const DevState &s, int g, const Ghost &G) {
+	s.rho[g] = G.rho;
+	s.u[g] = G.u;
+	s.v[g] = G.v;
+	s.w[g] = G.w;
+	s.p[g] = G.p;
+	s.E[g] = G.E;
+	s.H[g] = G.H;
+	s.M[g] = G.M;
 }
 
 // Solver::Roe, ConvectiveFluxesSchemes.cpp:10-165
@@ -32,7 +56,7 @@ __device__ __forceinline__ void roeFlux(const Prim &L, const Prim &R, double nx,
 	double wT = (L.w * sL + R.w * sR) / (sL + sR);
 	double hT = (L.H * sL + R.H * sR) / (sL + sR);
 	double q2 = uT * uT + vT * vT + wT * wT;
-	double cT = sqrt((gamma - 1.0) * (hT - q2 / 2));
+	double cT = sqrt((gamma - 1.0) * (hT - q2 * 0.5));
 	double VT = uT * nx + vT * ny + wT * nz;
 	double VL = L.u * nx + L.v * ny + L.w * nz;
 	double VR = R.u * nx + R.v * ny + R.w * nz;
@@ -93,25 +117,66 @@ __device__ __forceinline__ void ausmFlux(const Prim &L, const Prim &R, double nx
 	for (int i = 0; i < 5; i++) out[i] = mh * 0.5 * (phiL[i] + phiR[i]) - 0.5 * fabs(mh) * (phiR[i] - phiL[i]) + pn[i];
 }
 
-__device__ __forceinline__ void storeGhost(DevState &s, int g, double rho, double u, double v, double w, double M,
-                                           double p, double E, double H) {
-	s.rho[g] = rho;
-	s.u[g] = u;
-	s.v[g] = v;
-	s.w[g] = w;
-	s.M[g] = M;
-	s.p[g] = p;
-	s.E[g] = E;
-	s.H[g] = H;
+template<int SCHEME>
+__device__ __forceinline__ void schemeFlux(const Prim &L, const Prim &R, double nx, double ny, double nz, double g,
+                                           double out[5]) {
+	if (SCHEME == 1) ausmFlux(L, R, nx, ny, nz, g, out);
+	else roeFlux(L, R, nx, ny, nz, g, out);
 }
 
 // interior Mach number as FlowField::updateWRungeKutta stores it (FlowField.cpp:157, c/|u|)
-__device__ __forceinline__ double machOf(const DevState &s, int i, double g) {
-	double u = s.u[i], v = s.v[i], w = s.w[i];
-	return sqrt(g * (s.p[i] / s.rho[i])) / sqrt((u * u + v * v + w * w));
+__device__ __forceinline__ double machOf(double rho, double u, double v, double w, double p, double g) {
+	return sqrt(g * (p / rho)) / sqrt((u * u + v * v + w * w));
 }
 
-// EulerSolver::updateBC (EulerSolver.cpp:201-247) and PhysicalBC.cpp:11-231, one thread per ghost
+// Wall (PhysicalBC.cpp:124-162) and symmetry (:165-231): mirror the interior velocity
+__device__ __forceinline__ Ghost mirrorGhost(double ri, double ui, double vi, double wi, double pi, double nx, double ny,
+                                             double nz, double gm) {
+	double c = sqrt(gm * (pi / ri));
+	double V = ui * nx + vi * ny + wi * nz;
+	double u = ui - 2 * V * nx;
+	double v = vi - 2 * V * ny;
+	double w = wi - 2 * V * nz;
+	double M = sqrt(u * u + v * v + w * w) / c;
+	double E = pi / ((gm - 1) * ri) + ((u * u + v * v + w * w) * 0.5);
+	double H = E + (pi / ri);
+	return {ri, u, v, w, pi, E, H, M};
+}
+
+// Farfield: branch on the ghost's previous velocity and Mach (EulerSolver.cpp:205-226), then
+// PhysicalBC.cpp:11-121. `in` holds the ghost's previous state, interior cell i.
+__device__ __forceinline__ Ghost farfieldGhost(const DevState &in, int g, int i, double Mi, double nx, double ny,
+                                               double nz, const DevFree &q) {
+	double gm = q.gamma;
+	double Vg = in.u[g] * nx + in.v[g] * ny + in.w[g] * nz;
+	double Mg = in.M[g];
+	double ri = in.rho[i], pi = in.p[i], ui = in.u[i], vi = in.v[i], wi = in.w[i];
+	if (Mg >= 1.0 && Vg > 0.0) return {ri, ui, vi, wi, pi, in.E[i], in.H[i], Mi};
+	if (Mg >= 1.0 && Vg <= 0.0) return {q.rho_inf, q.u_inf, q.v_inf, q.w_inf, q.p_inf, q.E_inf, q.H_inf, q.M_inf};
+	double c = sqrt(gm * (pi / ri));
+	double p, rho, u, v, w;
+	if (Vg > 0.0) {
+		// subsonic outflow, PhysicalBC.cpp:82-121
+		p = q.p_inf;
+		rho = ri + (p - pi) / (c * c);
+		u = ui + nx * (pi - p) / (ri * c);
+		v = vi + ny * (pi - p) / (ri * c);
+		w = wi + nz * (pi - p) / (ri * c);
+	} else {
+		// subsonic inflow, PhysicalBC.cpp:43-79
+		p = 0.5 * (q.p_inf + pi - ri * c * ((nx * (q.u_inf - ui)) + (ny * (q.v_inf - vi)) + (nz * (q.w_inf - wi))));
+		rho = q.rho_inf + (p - q.p_inf) / (c * c);
+		u = q.u_inf - nx * (q.p_inf - p) / (ri * c);
+		v = q.v_inf - ny * (q.p_inf - p) / (ri * c);
+		w = q.w_inf - nz * (q.p_inf - p) / (ri * c);
+	}
+	double M = sqrt(u * u + v * v + w * w) / c;
+	double E = p / ((gm - 1) * rho) + ((u * u + v * v + w * w) * 0.5);
+	double H = E + (p / rho);
+	return {rho, u, v, w, p, E, H, M};
+}
+
+// EulerSolver::updateBC (EulerSolver.cpp:201-247), one thread per ghost, in place
 __global__ void __launch_bounds__(kBlock) kBc(DevMesh m, DevState s, DevFree q, int machDerived) {
 	int k = blockIdx.x * kBlock + threadIdx.x;
 	if (k >= m.nGhost) return;
@@ -119,54 +184,14 @@ __global__ void __launch_bounds__(kBlock) kBc(DevMesh m, DevState s, DevFree q, 
 	int g = m.nElem + k;
 	int i = m.f.e0[f];
 	double nx = m.f.nx[f], ny = m.f.ny[f], nz = m.f.nz[f];
-	double gm = q.gamma;
-	int kind = m.gkind[k];
-	if (kind == 0) {
-		double Vg = s.u[g] * nx + s.v[g] * ny + s.w[g] * nz;
-		double Mg = s.M[g];
-		if (Mg >= 1.0 && Vg > 0.0) {
-			double Mi = machDerived ? machOf(s, i, gm) : s.M[i];
-			storeGhost(s, g, s.rho[i], s.u[i], s.v[i], s.w[i], Mi, s.p[i], s.E[i], s.H[i]);
-		} else if (Mg < 1.0 && Vg > 0.0) {
-			double ri = s.rho[i], pi = s.p[i];
-			double c = sqrt(gm * (pi / ri));
-			double p = q.p_inf;
-			double rho = ri + (p - pi) / (c * c);
-			double u = s.u[i] + nx * (pi - p) / (ri * c);
-			double v = s.v[i] + ny * (pi - p) / (ri * c);
-			double w = s.w[i] + nz * (pi - p) / (ri * c);
-			double M = sqrt(u * u + v * v + w * w) / c;
-			double E = p / ((gm - 1) * rho) + ((u * u + v * v + w * w) / 2);
-			double H = E + (p / rho);
-			storeGhost(s, g, rho, u, v, w, M, p, E, H);
-		} else if (Mg >= 1.0 && Vg <= 0.0) {
-			storeGhost(s, g, q.rho_inf, q.u_inf, q.v_inf, q.w_inf, q.M_inf, q.p_inf, q.E_inf, q.H_inf);
-		} else if (Mg < 1.0 && Vg <= 0.0) {
-			double ri = s.rho[i], pi = s.p[i];
-			double c = sqrt(gm * (pi / ri));
-			double p = 0.5 * (q.p_inf + pi - ri * c * ((nx * (q.u_inf - s.u[i])) + (ny * (q.v_inf - s.v[i])) + (nz * (q.w_inf - s.w[i]))));
-			double rho = q.rho_inf + (p - q.p_inf) / (c * c);
-			double u = q.u_inf - nx * (q.p_inf - p) / (ri * c);
-			double v = q.v_inf - ny * (q.p_inf - p) / (ri * c);
-			double w = q.w_inf - nz * (q.p_inf - p) / (ri * c);
-			double M = sqrt(u * u + v * v + w * w) / c;
-			double E = p / ((gm - 1) * rho) + ((u * u + v * v + w * w) / 2);
-			double H = E + (p / rho);
-			storeGhost(s, g, rho, u, v, w, M, p, E, H);
-		}
+	Ghost G;
+	if (m.gkind[k] == 0) {
+		double Mi = machDerived ? machOf(s.rho[i], s.u[i], s.v[i], s.w[i], s.p[i], q.gamma) : s.M[i];
+		G = farfieldGhost(s, g, i, Mi, nx, ny, nz, q);
 	} else {
-		// wall (PhysicalBC.cpp:124-162) and symmetry (:165-231) mirror the interior velocity
-		double ri = s.rho[i], pi = s.p[i], ui = s.u[i], vi = s.v[i], wi = s.w[i];
-		double c = sqrt(gm * (pi / ri));
-		double V = ui * nx + vi * ny + wi * nz;
-		double u = ui - 2 * V * nx;
-		double v = vi - 2 * V * ny;
-		double w = wi - 2 * V * nz;
-		double M = sqrt(u * u + v * v + w * w) / c;
-		double E = pi / ((gm - 1) * ri) + ((u * u + v * v + w * w) / 2.0);
-		double H = E + (pi / ri);
-		storeGhost(s, g, ri, u, v, w, M, pi, E, H);
+		G = mirrorGhost(s.rho[i], s.u[i], s.v[i], s.w[i], s.p[i], nx, ny, nz, q.gamma);
 	}
+	storeGhost(s, g, G);
 }
 
 // per-face flux as EulerSolver::computeResidual dispatches it (EulerSolver.cpp:306-358)
@@ -264,7 +289,46 @@ __device__ __forceinline__ double blockSum(double x, double *lds) {
 	return lds[0];
 }
 
-// One RK stage for one cell: EulerSolver.cpp:431-449 / 459-470 and FlowField.cpp:140-158
+// RK stage update of one cell from its residual r: EulerSolver.cpp:431-449 / 459-470 and
+// FlowField.cpp:140-158. Reads the cell's state from `in`, writes `out`.
+__device__ __forceinline__ void updateCell(const DevMesh &m, const DevState &in, const DevState &out, const DevFree &q,
+                                           double *W0, double alpha, int firstStage, int e, const double r[5]) {
+	size_t n = (size_t) m.nElem;
+	double rho = in.rho[e], u = in.u[e], v = in.v[e], w = in.w[e], p = in.p[e];
+	double w0[5];
+	if (firstStage) {
+		double E = in.E[e];
+		w0[0] = rho;
+		w0[1] = u * rho;
+		w0[2] = v * rho;
+		w0[3] = w * rho;
+		w0[4] = E * rho;
+#pragma unroll
+		for (int k = 0; k < 5; k++) W0[k * n + e] = w0[k];
+	} else {
+#pragma unroll
+		for (int k = 0; k < 5; k++) w0[k] = W0[k * n + e];
+	}
+	double dt = localDt(m, q, rho, u, v, w, p, e);
+	double V = m.c.vol[e];
+	double nr = w0[0] - alpha * dt * r[0] / V;
+	double nru = w0[1] - alpha * dt * r[1] / V;
+	double nrv = w0[2] - alpha * dt * r[2] / V;
+	double nrw = w0[3] - alpha * dt * r[3] / V;
+	double nrE = w0[4] - alpha * dt * r[4] / V;
+	double nu = nru / nr, nv = nrv / nr, nw = nrw / nr, nE = nrE / nr;
+	double g = q.gamma;
+	double np = (g - 1) * nr * (nE - (nu * nu + nv * nv + nw * nw) * 0.5);
+	double nH = nE + (np / nr);
+	out.rho[e] = nr;
+	out.u[e] = nu;
+	out.v[e] = nv;
+	out.w[e] = nw;
+	out.E[e] = nE;
+	out.p[e] = np;
+	out.H[e] = nH;
+}
+
 __global__ void __launch_bounds__(kBlock) kStage(DevMesh m, DevState s, DevFree q, const double *fs, double *W0, double alpha,
                                                  int firstStage, double *rmsPartial, double *Rout) {
 	__shared__ double lds[kBlock];
@@ -278,44 +342,89 @@ __global__ void __launch_bounds__(kBlock) kStage(DevMesh m, DevState s, DevFree 
 #pragma unroll
 			for (int k = 0; k < 5; k++) Rout[k * n + e] = r[k];
 		}
-		double rho = s.rho[e], u = s.u[e], v = s.v[e], w = s.w[e], p = s.p[e];
-		double w0[5];
-		if (firstStage) {
-			double E = s.E[e];
-			w0[0] = rho;
-			w0[1] = u * rho;
-			w0[2] = v * rho;
-			w0[3] = w * rho;
-			w0[4] = E * rho;
-#pragma unroll
-			for (int k = 0; k < 5; k++) W0[k * n + e] = w0[k];
-		} else {
-#pragma unroll
-			for (int k = 0; k < 5; k++) w0[k] = W0[k * n + e];
-		}
-		double dt = localDt(m, q, rho, u, v, w, p, e);
-		double V = m.c.vol[e];
-		double nr = w0[0] - alpha * dt * r[0] / V;
-		double nru = w0[1] - alpha * dt * r[1] / V;
-		double nrv = w0[2] - alpha * dt * r[2] / V;
-		double nrw = w0[3] - alpha * dt * r[3] / V;
-		double nrE = w0[4] - alpha * dt * r[4] / V;
-		double nu = nru / nr, nv = nrv / nr, nw = nrw / nr, nE = nrE / nr;
-		double g = q.gamma;
-		double np = (g - 1) * nr * (nE - (nu * nu + nv * nv + nw * nw) / 2.0);
-		double nH = nE + (np / nr);
-		s.rho[e] = nr;
-		s.u[e] = nu;
-		s.v[e] = nv;
-		s.w[e] = nw;
-		s.E[e] = nE;
-		s.p[e] = np;
-		s.H[e] = nH;
+		updateCell(m, s, s, q, W0, alpha, firstStage, e, r);
 		r2 = r[0] * r[0];
 	}
 	if (rmsPartial) {
 		double t = blockSum(r2, lds);
 		if (threadIdx.x == 0) rmsPartial[blockIdx.x] = t;
+	}
+}
+
+// Fused RK stage over one block of cells (see the file comment).
+template<int SCHEME>
+__global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out, const double *Mint,
+                                                      DevFree q, double *W0, double alpha, int firstStage, int machDerived,
+                                                      double *rmsPartial) {
+	extern __shared__ double lds[];
+	const int b = blockIdx.x;
+	const int s0 = p.blockSlot[b], ns = p.blockSlot[b + 1] - s0;
+	const int ms = p.maxSlots;
+	// ---- face fluxes of the block, once each, into LDS
+	for (int j = threadIdx.x; j < ns; j += kBlock) {
+		const int s = s0 + j;
+		const int a = p.e0[s], c = p.e1[s];
+		const int kind = p.kind[s];
+		const double nx = p.nx[s], ny = p.ny[s], nz = p.nz[s], S = p.S[s];
+		const Prim L = loadPrim(in, a);
+		double F[5];
+		if (kind == 0) {
+			schemeFlux<SCHEME>(L, loadPrim(in, c), nx, ny, nz, q.gamma, F);
+		} else {
+			Ghost G;
+			if (kind == 3) {
+				double Mi = machDerived ? machOf(L.rho, L.u, L.v, L.w, L.p, q.gamma) : Mint[a];
+				G = farfieldGhost(in, c, a, Mi, nx, ny, nz, q);
+			} else {
+				G = mirrorGhost(L.rho, L.u, L.v, L.w, L.p, nx, ny, nz, q.gamma);
+			}
+			storeGhost(out, c, G);
+			if (kind == 1) {
+				F[0] = 0;
+				F[1] = nx * G.p;
+				F[2] = ny * G.p;
+				F[3] = nz * G.p;
+				F[4] = 0;
+			} else if (kind == 2) {
+				schemeFlux<SCHEME>(L, Prim{G.rho, G.u, G.v, G.w, G.p, G.H}, nx, ny, nz, q.gamma, F);
+			} else {
+				double V = G.u * nx + G.v * ny + G.w * nz;
+				F[0] = G.rho * V;
+				F[1] = G.rho * G.u * V + G.p * nx;
+				F[2] = G.rho * G.v * V + G.p * ny;
+				F[3] = G.rho * G.w * V + G.p * nz;
+				F[4] = G.rho * G.H * V;
+			}
+		}
+#pragma unroll
+		for (int k = 0; k < 5; k++) lds[k * ms + j] = F[k] * S;
+	}
+	__syncthreads();
+	// ---- one thread per cell: gather in reference order, local dt, update
+	const int e = b * kBlock + threadIdx.x;
+	double r2 = 0.0;
+	if (e < m.nElem) {
+		double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+		const uint16_t *cs = p.csr + (size_t) e * p.csrStride;
+		for (int i = 0; i < p.csrStride; i++) {
+			const unsigned code = cs[i];
+			if (code == 0xFFFFu) break;
+			const int j = (int) (code & 0x7FFFu);
+			if (code & 0x8000u) {
+#pragma unroll
+				for (int k = 0; k < 5; k++) r[k] -= lds[k * ms + j];
+			} else {
+#pragma unroll
+				for (int k = 0; k < 5; k++) r[k] += lds[k * ms + j];
+			}
+		}
+		updateCell(m, in, out, q, W0, alpha, firstStage, e, r);
+		r2 = r[0] * r[0];
+	}
+	if (rmsPartial) {
+		__syncthreads();
+		double t = blockSum(r2, lds);
+		if (threadIdx.x == 0) rmsPartial[b] = t;
 	}
 }
 
@@ -333,7 +442,7 @@ __global__ void __launch_bounds__(kBlock) kSum(const double *in, int n, double *
 __global__ void __launch_bounds__(kBlock) kMach(DevMesh m, DevState s, DevFree q) {
 	int e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= m.nElem) return;
-	s.M[e] = machOf(s, e, q.gamma);
+	s.M[e] = machOf(s.rho[e], s.u[e], s.v[e], s.w[e], s.p[e], q.gamma);
 }
 
 // AeroCoefficients::Update (AeroCoefficients.cpp:38-52): per wall face coeff = N*cp*Cref/Sref and
@@ -396,6 +505,19 @@ void launchStage(const DevMesh &m, DevState s, const DevFree &q, const double *f
 	if (m.nElem == 0) return;
 	hipLaunchKernelGGL(kStage, dim3(blocksFor(m.nElem)), dim3(kBlock), 0, st, m, s, q, fs, W0, alpha, firstStage,
 	                   rmsPartial, Rout);
+}
+
+void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const double *Mint,
+                      const DevFree &q, int scheme, double *W0, double alpha, int firstStage, int machDerived,
+                      double *rmsPartial, hipStream_t st) {
+	if (p.nBlocks == 0) return;
+	size_t lds = sizeof(double) * 5 * (size_t) (p.maxSlots > kBlock ? p.maxSlots : kBlock);
+	if (scheme == 1)
+		hipLaunchKernelGGL(kStageFused<1>, dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, Mint, q, W0, alpha,
+		                   firstStage, machDerived, rmsPartial);
+	else
+		hipLaunchKernelGGL(kStageFused<0>, dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, Mint, q, W0, alpha,
+		                   firstStage, machDerived, rmsPartial);
 }
 
 void launchSum(const double *in, int n, double *out, hipStream_t st) {

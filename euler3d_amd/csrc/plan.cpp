@@ -1,0 +1,188 @@
+// Device data plan: locality order of the cells, fused-stage blocks and device-numbered arrays.
+//
+// The device keeps cells in a Morton (Z-curve) order of their centroids, so kCellsPerBlock
+// consecutive cells form a compact cluster whose faces are mostly shared inside the cluster.
+// The reference numbering is kept where it determines the FP64 bits (face orientation and the
+// per-cell order of accumulation, ascending reference face id, EulerSolver.cpp:306-358), so the
+// order changes memory locality only.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <stdexcept>
+
+namespace e3d {
+
+namespace {
+
+uint64_t spread3(uint64_t x) {
+	x &= 0x1fffff;
+	x = (x | x << 32) & 0x1f00000000ffffULL;
+	x = (x | x << 16) & 0x1f0000ff0000ffULL;
+	x = (x | x << 8) & 0x100f00f00f00f00fULL;
+	x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+	x = (x | x << 2) & 0x1249249249249249ULL;
+	return x;
+}
+
+std::vector<int> mortonOrder(const SolverMesh &sm) {
+	const int n = sm.nElem;
+	double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+	for (int e = 0; e < n; e++)
+		for (int k = 0; k < 3; k++) {
+			double x = sm.cellCentroid[3 * (size_t) e + k];
+			lo[k] = std::min(lo[k], x);
+			hi[k] = std::max(hi[k], x);
+		}
+	double ext = 0;
+	for (int k = 0; k < 3; k++) ext = std::max(ext, hi[k] - lo[k]);
+	double scale = ext > 0 ? 2097151.0 / ext : 0.0;
+	std::vector<uint64_t> key(n);
+	for (int e = 0; e < n; e++) {
+		uint64_t q[3];
+		for (int k = 0; k < 3; k++) {
+			double t = (sm.cellCentroid[3 * (size_t) e + k] - lo[k]) * scale;
+			q[k] = (uint64_t) std::min(2097151.0, std::max(0.0, t));
+		}
+		key[e] = spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2;
+	}
+	std::vector<int> order(n);
+	std::iota(order.begin(), order.end(), 0);
+	std::sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b] || (key[a] == key[b] && a < b); });
+	return order;
+}
+
+int8_t slotKind(const SolverMesh &sm, int f) {
+	if (f < sm.nFaceInt) return 0;
+	switch (sm.ghostKind[f - sm.nFaceInt]) {
+		case BND_WALL: return 1;
+		case BND_SYM: return 2;
+		default: return 3;
+	}
+}
+
+}// namespace
+
+HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock) {
+	HostPlan P;
+	const int nE = sm.nElem, nF = sm.nFace;
+	P.nElem = nE;
+	P.nFace = nF;
+	P.nFaceInt = sm.nFaceInt;
+	P.nGhost = sm.nGhost;
+	if (order == ORDER_MORTON) {
+		P.new2old = mortonOrder(sm);
+	} else {
+		P.new2old.resize(nE);
+		std::iota(P.new2old.begin(), P.new2old.end(), 0);
+	}
+	P.old2new.assign(nE, -1);
+	for (int i = 0; i < nE; i++) P.old2new[P.new2old[i]] = i;
+	auto dev = [&](int cell) { return cell < nE ? P.old2new[cell] : cell; };
+
+	// faces in reference order
+	P.fe0.resize(nF), P.fe1.resize(nF);
+	P.fnx.resize(nF), P.fny.resize(nF), P.fnz.resize(nF), P.fS.resize(nF);
+	for (int f = 0; f < nF; f++) {
+		P.fe0[f] = dev(sm.face2element[2 * (size_t) f]);
+		P.fe1[f] = dev(sm.face2element[2 * (size_t) f + 1]);
+		P.fnx[f] = sm.faceUnitNormal[3 * (size_t) f];
+		P.fny[f] = sm.faceUnitNormal[3 * (size_t) f + 1];
+		P.fnz[f] = sm.faceUnitNormal[3 * (size_t) f + 2];
+		P.fS[f] = sm.faceArea[f];
+	}
+	P.gkind.resize(sm.nGhost);
+	for (int k = 0; k < sm.nGhost; k++)
+		P.gkind[k] = (int8_t) (sm.ghostKind[k] == BND_WALL ? 1 : sm.ghostKind[k] == BND_SYM ? 2 : 0);
+
+	// cells in device order: static time-step sums (local-face order, TimeIntegration.cpp:20-26)
+	// and the face list in ascending reference face id
+	P.vol.resize(nE), P.sx.resize(nE), P.sy.resize(nE), P.sz.resize(nE);
+	P.fstart.assign(nE + 1, 0);
+	int maxFaces = 0;
+	for (int c = 0; c < nE; c++) {
+		int old = P.new2old[c];
+		int b = sm.element2faceStart[old], en = sm.element2faceStart[old + 1];
+		maxFaces = std::max(maxFaces, en - b);
+		P.fstart[c + 1] = P.fstart[c] + (en - b);
+	}
+	P.flist.resize(P.fstart[nE]);
+	for (int c = 0; c < nE; c++) {
+		int old = P.new2old[c];
+		int b = sm.element2faceStart[old], en = sm.element2faceStart[old + 1];
+		double ax = 0, ay = 0, az = 0;
+		for (int j = b; j < en; j++) {
+			int f = sm.element2face[j];
+			ax += std::fabs(sm.faceNormal[3 * (size_t) f]);
+			ay += std::fabs(sm.faceNormal[3 * (size_t) f + 1]);
+			az += std::fabs(sm.faceNormal[3 * (size_t) f + 2]);
+		}
+		P.vol[c] = sm.cellVolume[old];
+		P.sx[c] = ax, P.sy[c] = ay, P.sz[c] = az;
+		int fl[8];
+		for (int j = b; j < en; j++) fl[j - b] = sm.element2face[j];
+		std::sort(fl, fl + (en - b));
+		for (int j = 0; j < en - b; j++) {
+			int f = fl[j];
+			bool second = sm.face2element[2 * (size_t) f + 1] == old;
+			P.flist[P.fstart[c] + j] = second ? (int) ((unsigned) f | 0x80000000u) : f;
+		}
+	}
+
+	// fused-stage blocks: every face touching the block gets one slot in it
+	P.cellsPerBlock = cellsPerBlock;
+	P.nBlocks = (nE + cellsPerBlock - 1) / cellsPerBlock;
+	P.csrStride = maxFaces;
+	P.csr.assign((size_t) nE * maxFaces, 0xFFFF);
+	P.blockSlot.assign(P.nBlocks + 1, 0);
+	std::vector<int> slotOf(nF, -1), slotBlock(nF, -1);
+	size_t reserve = (size_t) nF + nF / 2;
+	P.se0.reserve(reserve), P.se1.reserve(reserve), P.skind.reserve(reserve);
+	P.snx.reserve(reserve), P.sny.reserve(reserve), P.snz.reserve(reserve), P.sS.reserve(reserve);
+	for (int blk = 0; blk < P.nBlocks; blk++) {
+		int c0 = blk * cellsPerBlock, c1 = std::min(nE, c0 + cellsPerBlock);
+		int ns = 0;
+		for (int c = c0; c < c1; c++) {
+			for (int j = P.fstart[c]; j < P.fstart[c + 1]; j++) {
+				unsigned code = (unsigned) P.flist[j];
+				int f = (int) (code & 0x7fffffffu);
+				int slot;
+				if (slotBlock[f] == blk) {
+					slot = slotOf[f];
+				} else {
+					slot = ns++;
+					slotBlock[f] = blk;
+					slotOf[f] = slot;
+					P.se0.push_back(P.fe0[f]);
+					P.se1.push_back(P.fe1[f]);
+					P.skind.push_back(slotKind(sm, f));
+					P.snx.push_back(P.fnx[f]);
+					P.sny.push_back(P.fny[f]);
+					P.snz.push_back(P.fnz[f]);
+					P.sS.push_back(P.fS[f]);
+				}
+				if (slot > 0x7FFF) throw std::runtime_error("fused-stage block has more than 32768 face slots");
+				P.csr[(size_t) c * maxFaces + (j - P.fstart[c])] = (uint16_t) (slot | ((code >> 31) << 15));
+			}
+		}
+		P.blockSlot[blk + 1] = P.blockSlot[blk] + ns;
+		P.maxSlots = std::max(P.maxSlots, ns);
+	}
+
+	// wall list for the force integration, in the reference's order (Mesh.hpp:594-627)
+	size_t nw = sm.wallGhost.size();
+	P.wallCell.resize(nw);
+	for (int k = 0; k < 3; k++) P.wallN[k].resize(nw), P.wallC[k].resize(nw);
+	for (size_t i = 0; i < nw; i++) {
+		P.wallCell[i] = P.old2new[sm.wallCell[i]];
+		int f = sm.wallFace[i];
+		for (int k = 0; k < 3; k++) {
+			P.wallN[k][i] = sm.faceNormal[3 * (size_t) f + k];
+			P.wallC[k][i] = sm.faceCenter[3 * (size_t) f + k];
+		}
+	}
+	return P;
+}
+
+}// namespace e3d

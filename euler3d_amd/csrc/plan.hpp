@@ -1,0 +1,42 @@
+// Device data plan of one case: a locality order of the cells, the block structure of the fused
+// stage kernel, and every per-face / per-cell array in the device numbering.
+#pragma once
+
+#include "e3d_host.hpp"
+
+#include <cstdint>
+#include <vector>
+
+namespace e3d {
+
+enum CellOrder : int { ORDER_REFERENCE = 0, ORDER_MORTON = 1 };
+
+struct HostPlan {
+	int nElem = 0, nFace = 0, nFaceInt = 0, nGhost = 0;
+	std::vector<int> new2old, old2new; // interior cells: device id <-> reference id
+
+	// per-face arrays in reference face order, cell ids in device numbering
+	std::vector<int> fe0, fe1;
+	std::vector<double> fnx, fny, fnz, fS;
+	std::vector<int8_t> gkind; // per ghost: 0 farfield, 1 wall, 2 symmetry
+
+	// per-cell arrays in device order
+	std::vector<double> vol, sx, sy, sz;
+	std::vector<int> fstart, flist; // reference face ids ascending, bit 31 = subtract
+
+	// fused-stage block plan
+	int cellsPerBlock = 0, nBlocks = 0, maxSlots = 0, csrStride = 0;
+	std::vector<int> blockSlot;
+	std::vector<int> se0, se1;
+	std::vector<int8_t> skind;
+	std::vector<double> snx, sny, snz, sS;
+	std::vector<uint16_t> csr;
+
+	// wall list (reference order) for the force integration
+	std::vector<int> wallCell;
+	std::vector<double> wallN[3], wallC[3];
+};
+
+HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock);
+
+}// namespace e3d
