@@ -64,6 +64,9 @@ def main() -> None:
     ap.add_argument("--nz", type=int, default=64)
     ap.add_argument("--scheme", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="first run one plain device-to-device copy of this many MiB (a known-size kernel "
+                         "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs one resident mesh per GPU and measures one GPU only; "
@@ -71,6 +74,16 @@ def main() -> None:
         sys.exit(2)
 
     from euler3d_amd import EulerSolver, Mesh, default_params
+
+    if a.copy_calib_mib:
+        import torch
+
+        src = torch.ones(a.copy_calib_mib << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {a.copy_calib_mib} MiB read + {a.copy_calib_mib} MiB written")
+        del src, dst
 
     t0 = time.perf_counter()
     mesh = Mesh.bump(nx=a.nx, ny=a.ny, nz=a.nz, jitter=0.1, seed=0, permute=True)
