@@ -89,6 +89,7 @@ struct e3d_ctx {
 	std::vector<hipEvent_t> evPool;
 	size_t evUsed = 0;
 	bool fused = true;       // fused stage kernel (off: the three-pass kBc/kFlux/kStage form)
+	e3d::StageTuning tune{};
 	std::vector<int> new2old; // device cell id -> reference cell id (interior)
 	// mesh, device numbering
 	DBuf<int> e0, e1, fstart, flist;
@@ -171,7 +172,7 @@ void runStage(e3d_ctx *c, int stage, double *rmsPartial) {
 		auto o = c->state(c->cur ^ 1);
 		record(evA);
 		e3d::launchStageFused(c->dm, c->plan, s, o, s.M, c->fr, c->prm.scheme, c->W0.p, alphas[stage], stage == 0,
-		                      c->machDerived, rmsPartial, c->stream);
+		                      c->machDerived, rmsPartial, c->tune, c->stream);
 		record(evB);
 		c->cur ^= 1;
 	} else {
@@ -454,6 +455,10 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		const char *pathEnv = std::getenv("E3D_STAGE_PATH");
 		int order = (orderEnv && std::string(orderEnv) == "reference") ? e3d::ORDER_REFERENCE : e3d::ORDER_MORTON;
 		c->fused = !(pathEnv && std::string(pathEnv) == "unfused");
+		// launch-shape overrides for measurement (results are identical for every choice)
+		if (const char *v = std::getenv("E3D_LDS_STATE")) c->tune.ldsState = std::atoi(v) != 0;
+		if (const char *v = std::getenv("E3D_NONTEMPORAL")) c->tune.nontemporal = std::atoi(v) != 0;
+		if (const char *v = std::getenv("E3D_XCDS")) c->tune.xcds = std::max(1, std::atoi(v));
 		e3d::HostPlan P = e3d::buildPlan(sm, order, e3d::kCellsPerBlock);
 		if ((size_t) P.maxSlots * 5 * sizeof(double) > 160 * 1024)
 			return fail(E3D_ERR_UNSUPPORTED, "a block of the fused stage needs more LDS than a CU has");

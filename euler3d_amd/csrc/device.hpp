@@ -79,11 +79,18 @@ void launchTimestep(const DevMesh &m, DevState s, const DevFree &q, double *dt, 
 void launchStage(const DevMesh &m, DevState s, const DevFree &q, const double *fs, double *W0, double alpha,
                  int firstStage, double *rmsPartial, double *Rout, hipStream_t st);
 int stageBlocks(int nElem);
+// launch choices of the fused stage (they change speed only, never results)
+struct StageTuning {
+	bool ldsState = true;    // stage the block's own cells in LDS
+	bool nontemporal = true; // non-temporal loads for arrays read once per stage
+	int xcds = 8;            // XCD count for the L2-aware block order (1: plain dispatch order)
+};
+
 // Fused RK stage: boundary ghosts, face fluxes (once per block, in LDS), residual gather, local
 // dt and update, reading `in` and writing `out` (interior and ghosts).
 void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const double *Mint,
                       const DevFree &q, int scheme, double *W0, double alpha, int firstStage, int machDerived,
-                      double *rmsPartial, hipStream_t st);
+                      double *rmsPartial, const StageTuning &t, hipStream_t st);
 // deterministic sum of n doubles into out[0] (single workgroup, fixed order)
 void launchSum(const double *in, int n, double *out, hipStream_t st);
 // interior Mach number from the primitives (FlowField.cpp:157), for state read-back

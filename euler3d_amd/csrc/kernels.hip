@@ -21,6 +21,9 @@ namespace e3d {
 namespace {
 
 constexpr int kBlock = 256;
+#ifndef E3D_FUSED_WAVES
+#define E3D_FUSED_WAVES 1
+#endif
 static_assert(kBlock == kCellsPerBlock, "one thread per cell in the fused stage");
 
 struct Prim {
@@ -30,6 +33,14 @@ struct Prim {
 struct Ghost {
 	double rho, u, v, w, p, E, H, M;
 };
+
+// streamed (read-once-per-stage) arrays: optionally loaded non-temporally so that they do not
+// evict the cell state that neighbouring faces and blocks re-read from L2
+template<bool NT, typename T>
+__device__ __forceinline__ T ldStream(const T *p) {
+	if constexpr (NT) return __builtin_nontemporal_load(p);
+	else return *p;
+}
 
 __device__ __forceinline__ Prim loadPrim(const DevState &s, int i) {
 	return {s.rho[i], s.u[i], s.v[i], s.w[i], s.p[i], s.H[i]};
@@ -91,6 +102,52 @@ __device__ __forceinline__ void roeFlux(const Prim &L, const Prim &R, double nx,
 	}
 }
 
+// The same Roe flux, component by component: each of the five values is formed with exactly the
+// operations of roeFlux and handed to emit(k, F_k) at once, so no 5-vector of temporaries is live
+// (fewer VGPRs, more resident waves in the fused stage).
+template<typename Emit>
+__device__ __forceinline__ void roeFluxEmit(const Prim &L, const Prim &R, double nx, double ny, double nz, double gamma,
+                                            Emit emit) {
+	double sL = sqrt(L.rho), sR = sqrt(R.rho);
+	double rhoT = sqrt(L.rho * R.rho);
+	double uT = (L.u * sL + R.u * sR) / (sL + sR);
+	double vT = (L.v * sL + R.v * sR) / (sL + sR);
+	double wT = (L.w * sL + R.w * sR) / (sL + sR);
+	double hT = (L.H * sL + R.H * sR) / (sL + sR);
+	double q2 = uT * uT + vT * vT + wT * wT;
+	double cT = sqrt((gamma - 1.0) * (hT - q2 * 0.5));
+	double VT = uT * nx + vT * ny + wT * nz;
+	double VL = L.u * nx + L.v * ny + L.w * nz;
+	double VR = R.u * nx + R.v * ny + R.w * nz;
+	double dP = R.p - L.p, dVc = VR - VL;
+	double localC = sqrt(gamma * L.p / L.rho);
+	double delta = (1.0 / 15.0) * localC;
+	double a1 = VT - cT, a5 = VT + cT;
+	double l1 = fabs(a1) > delta ? fabs(a1) : (a1 * a1 + delta * delta) / (2.0 * delta);
+	double l5 = fabs(a5) > delta ? fabs(a5) : (a5 * a5 + delta * delta) / (2.0 * delta);
+	double c2x2 = 2.0 * (cT * cT);
+	double k1 = dP - rhoT * cT * dVc;
+	double k5 = dP + rhoT * cT * dVc;
+	double kr = (R.rho - L.rho) - dP / (cT * cT);
+	double aV = fabs(VT);
+	double rVR = R.rho * VR, lVL = L.rho * VL;
+	auto comp = [&](int k, double avg, double e1, double e2, double t2, double e5) {
+		double f1 = e1 * l1 * k1 / c2x2;
+		double f234 = (e2 * kr + t2) * aV;
+		double f5 = e5 * l5 * k5 / c2x2;
+		emit(k, avg - 0.5 * (f1 + f234 + f5));
+	};
+	comp(0, (rVR + lVL) * 0.5, 1.0, 1.0, 0.0, 1.0);
+	comp(1, ((rVR * R.u + R.p * nx) + (lVL * L.u + L.p * nx)) * 0.5, uT - cT * nx, uT, rhoT * ((R.u - L.u) - dVc * nx),
+	     uT + cT * nx);
+	comp(2, ((rVR * R.v + R.p * ny) + (lVL * L.v + L.p * ny)) * 0.5, vT - cT * ny, vT, rhoT * ((R.v - L.v) - dVc * ny),
+	     vT + cT * ny);
+	comp(3, ((rVR * R.w + R.p * nz) + (lVL * L.w + L.p * nz)) * 0.5, wT - cT * nz, wT, rhoT * ((R.w - L.w) - dVc * nz),
+	     wT + cT * nz);
+	comp(4, (rVR * R.H + lVL * L.H) * 0.5, hT - cT * VT, q2 * 0.5,
+	     rhoT * (uT * (R.u - L.u) + vT * (R.v - L.v) + wT * (R.w - L.w) - VT * dVc), hT + cT * VT);
+}
+
 // AUSM, Liou & Steffen, J. Comput. Phys. 107 (1993) 23-39 (the reference only declares it,
 // ConvectiveFluxesSchemes.h:27-30)
 __device__ __forceinline__ double machPlus(double M) { return fabs(M) <= 1.0 ? 0.25 * (M + 1.0) * (M + 1.0) : 0.5 * (M + fabs(M)); }
@@ -144,14 +201,15 @@ __device__ __forceinline__ Ghost mirrorGhost(double ri, double ui, double vi, do
 }
 
 // Farfield: branch on the ghost's previous velocity and Mach (EulerSolver.cpp:205-226), then
-// PhysicalBC.cpp:11-121. `in` holds the ghost's previous state, interior cell i.
-__device__ __forceinline__ Ghost farfieldGhost(const DevState &in, int g, int i, double Mi, double nx, double ny,
-                                               double nz, const DevFree &q) {
+// PhysicalBC.cpp:11-121. `in` holds the ghost's previous state; I, Ei, Mi are the interior
+// cell's primitives, energy and stored Mach number.
+__device__ __forceinline__ Ghost farfieldGhost(const DevState &in, int g, double Mi, double nx, double ny, double nz,
+                                               const DevFree &q, const Prim &I, double Ei) {
 	double gm = q.gamma;
 	double Vg = in.u[g] * nx + in.v[g] * ny + in.w[g] * nz;
 	double Mg = in.M[g];
-	double ri = in.rho[i], pi = in.p[i], ui = in.u[i], vi = in.v[i], wi = in.w[i];
-	if (Mg >= 1.0 && Vg > 0.0) return {ri, ui, vi, wi, pi, in.E[i], in.H[i], Mi};
+	double ri = I.rho, pi = I.p, ui = I.u, vi = I.v, wi = I.w;
+	if (Mg >= 1.0 && Vg > 0.0) return {ri, ui, vi, wi, pi, Ei, I.H, Mi};
 	if (Mg >= 1.0 && Vg <= 0.0) return {q.rho_inf, q.u_inf, q.v_inf, q.w_inf, q.p_inf, q.E_inf, q.H_inf, q.M_inf};
 	double c = sqrt(gm * (pi / ri));
 	double p, rho, u, v, w;
@@ -187,7 +245,7 @@ __global__ void __launch_bounds__(kBlock) kBc(DevMesh m, DevState s, DevFree q, 
 	Ghost G;
 	if (m.gkind[k] == 0) {
 		double Mi = machDerived ? machOf(s.rho[i], s.u[i], s.v[i], s.w[i], s.p[i], q.gamma) : s.M[i];
-		G = farfieldGhost(s, g, i, Mi, nx, ny, nz, q);
+		G = farfieldGhost(s, g, Mi, nx, ny, nz, q, loadPrim(s, i), s.E[i]);
 	} else {
 		G = mirrorGhost(s.rho[i], s.u[i], s.v[i], s.w[i], s.p[i], nx, ny, nz, q.gamma);
 	}
@@ -263,13 +321,14 @@ __global__ void __launch_bounds__(kBlock) kGather(DevMesh m, const double *fs, d
 
 // TimeStepMethod2, TimeIntegration.cpp:8-40 (sum|N| over the cell's faces is static: precomputed
 // on the host in the same local-face order)
+template<bool NT = false>
 __device__ __forceinline__ double localDt(const DevMesh &m, const DevFree &q, double rho, double u, double v, double w,
                                           double p, int e) {
 	double c = sqrt(q.gamma * p / rho);
-	double lx = 0.5 * (fabs(u) + c) * m.c.sx[e];
-	double ly = 0.5 * (fabs(v) + c) * m.c.sy[e];
-	double lz = 0.5 * (fabs(w) + c) * m.c.sz[e];
-	return q.cfl * m.c.vol[e] / (lx + ly + lz);
+	double lx = 0.5 * (fabs(u) + c) * ldStream<NT>(m.c.sx + e);
+	double ly = 0.5 * (fabs(v) + c) * ldStream<NT>(m.c.sy + e);
+	double lz = 0.5 * (fabs(w) + c) * ldStream<NT>(m.c.sz + e);
+	return q.cfl * ldStream<NT>(m.c.vol + e) / (lx + ly + lz);
 }
 
 __global__ void __launch_bounds__(kBlock) kTimestep(DevMesh m, DevState s, DevFree q, double *dt) {
@@ -290,14 +349,14 @@ __device__ __forceinline__ double blockSum(double x, double *lds) {
 }
 
 // RK stage update of one cell from its residual r: EulerSolver.cpp:431-449 / 459-470 and
-// FlowField.cpp:140-158. Reads the cell's state from `in`, writes `out`.
-__device__ __forceinline__ void updateCell(const DevMesh &m, const DevState &in, const DevState &out, const DevFree &q,
-                                           double *W0, double alpha, int firstStage, int e, const double r[5]) {
+// FlowField.cpp:140-158. (rho, u, v, w, p, E) is the cell's state before the stage; writes `out`.
+template<bool NT = false>
+__device__ __forceinline__ void updateCell(const DevMesh &m, const DevState &out, const DevFree &q, double *W0,
+                                           double alpha, int firstStage, int e, const double r[5], double rho, double u,
+                                           double v, double w, double p, double E) {
 	size_t n = (size_t) m.nElem;
-	double rho = in.rho[e], u = in.u[e], v = in.v[e], w = in.w[e], p = in.p[e];
 	double w0[5];
 	if (firstStage) {
-		double E = in.E[e];
 		w0[0] = rho;
 		w0[1] = u * rho;
 		w0[2] = v * rho;
@@ -307,10 +366,10 @@ __device__ __forceinline__ void updateCell(const DevMesh &m, const DevState &in,
 		for (int k = 0; k < 5; k++) W0[k * n + e] = w0[k];
 	} else {
 #pragma unroll
-		for (int k = 0; k < 5; k++) w0[k] = W0[k * n + e];
+		for (int k = 0; k < 5; k++) w0[k] = ldStream<NT>(W0 + k * n + e);
 	}
-	double dt = localDt(m, q, rho, u, v, w, p, e);
-	double V = m.c.vol[e];
+	double dt = localDt<NT>(m, q, rho, u, v, w, p, e);
+	double V = ldStream<NT>(m.c.vol + e);
 	double nr = w0[0] - alpha * dt * r[0] / V;
 	double nru = w0[1] - alpha * dt * r[1] / V;
 	double nrv = w0[2] - alpha * dt * r[2] / V;
@@ -342,7 +401,7 @@ __global__ void __launch_bounds__(kBlock) kStage(DevMesh m, DevState s, DevFree 
 #pragma unroll
 			for (int k = 0; k < 5; k++) Rout[k * n + e] = r[k];
 		}
-		updateCell(m, s, s, q, W0, alpha, firstStage, e, r);
+		updateCell(m, s, q, W0, alpha, firstStage, e, r, s.rho[e], s.u[e], s.v[e], s.w[e], s.p[e], s.E[e]);
 		r2 = r[0] * r[0];
 	}
 	if (rmsPartial) {
@@ -351,30 +410,76 @@ __global__ void __launch_bounds__(kBlock) kStage(DevMesh m, DevState s, DevFree 
 	}
 }
 
-// Fused RK stage over one block of cells (see the file comment).
-template<int SCHEME>
-__global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out, const double *Mint,
+// Workgroups are dispatched round-robin over the XCDs (each with its own L2). With xcds > 1 the
+// dispatch index is remapped so that every XCD runs a contiguous range of blocks: neighbouring
+// blocks, which share cut faces and read each other's cells, then share an L2.
+__device__ __forceinline__ int logicalBlock(int w, int nBlocks, int xcds) {
+	if (xcds <= 1) return w;
+	const int x = w % xcds, k = w / xcds;
+	const int q = nBlocks / xcds, r = nBlocks % xcds;
+	return x * q + min(x, r) + k;
+}
+
+// Fused RK stage over one block of cells (see the file comment). LDSSTATE stages the block's own
+// cells (rho, u, v, w, p, E, H) in LDS first; faces and the update then read them from LDS and
+// only neighbours across the block boundary come from global memory.
+template<int SCHEME, bool LDSSTATE, bool NT>
+__global__ void __launch_bounds__(kBlock, E3D_FUSED_WAVES) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out, const double *Mint,
                                                       DevFree q, double *W0, double alpha, int firstStage, int machDerived,
-                                                      double *rmsPartial) {
+                                                      double *rmsPartial, int xcds) {
 	extern __shared__ double lds[];
-	const int b = blockIdx.x;
+	const int b = logicalBlock(blockIdx.x, p.nBlocks, xcds);
 	const int s0 = p.blockSlot[b], ns = p.blockSlot[b + 1] - s0;
 	const int ms = p.maxSlots;
+	const int c0 = b * kBlock;
+	const int nc = min(kBlock, m.nElem - c0);
+	double *cs = lds + 5 * ms; // [7][kBlock] own-cell state when LDSSTATE
+	if constexpr (LDSSTATE) {
+		const int t = threadIdx.x;
+		if (t < nc) {
+			const int e = c0 + t;
+			cs[0 * kBlock + t] = in.rho[e];
+			cs[1 * kBlock + t] = in.u[e];
+			cs[2 * kBlock + t] = in.v[e];
+			cs[3 * kBlock + t] = in.w[e];
+			cs[4 * kBlock + t] = in.p[e];
+			cs[5 * kBlock + t] = in.E[e];
+			cs[6 * kBlock + t] = in.H[e];
+		}
+		__syncthreads();
+	}
+	auto prim = [&](int i) -> Prim {
+		if constexpr (LDSSTATE) {
+			const unsigned l = (unsigned) (i - c0);
+			if (l < (unsigned) nc)
+				return {cs[l], cs[kBlock + l], cs[2 * kBlock + l], cs[3 * kBlock + l], cs[4 * kBlock + l], cs[6 * kBlock + l]};
+		}
+		return loadPrim(in, i);
+	};
 	// ---- face fluxes of the block, once each, into LDS
 	for (int j = threadIdx.x; j < ns; j += kBlock) {
 		const int s = s0 + j;
-		const int a = p.e0[s], c = p.e1[s];
-		const int kind = p.kind[s];
-		const double nx = p.nx[s], ny = p.ny[s], nz = p.nz[s], S = p.S[s];
-		const Prim L = loadPrim(in, a);
+		const int a = ldStream<NT>(p.e0 + s), c = ldStream<NT>(p.e1 + s);
+		const int kind = ldStream<NT>(p.kind + s);
+		const double nx = ldStream<NT>(p.nx + s), ny = ldStream<NT>(p.ny + s), nz = ldStream<NT>(p.nz + s);
+		const double S = ldStream<NT>(p.S + s);
+		const Prim L = prim(a);
+		auto emit = [&](int k, double f) { lds[k * ms + j] = f * S; };
+		if (SCHEME == 0 && kind == 0) {
+			roeFluxEmit(L, prim(c), nx, ny, nz, q.gamma, emit);
+			continue;
+		}
 		double F[5];
 		if (kind == 0) {
-			schemeFlux<SCHEME>(L, loadPrim(in, c), nx, ny, nz, q.gamma, F);
+			schemeFlux<SCHEME>(L, prim(c), nx, ny, nz, q.gamma, F);
 		} else {
 			Ghost G;
 			if (kind == 3) {
 				double Mi = machDerived ? machOf(L.rho, L.u, L.v, L.w, L.p, q.gamma) : Mint[a];
-				G = farfieldGhost(in, c, a, Mi, nx, ny, nz, q);
+				double Ea;
+				if constexpr (LDSSTATE) Ea = cs[5 * kBlock + (a - c0)];
+				else Ea = in.E[a];
+				G = farfieldGhost(in, c, Mi, nx, ny, nz, q, L, Ea);
 			} else {
 				G = mirrorGhost(L.rho, L.u, L.v, L.w, L.p, nx, ny, nz, q.gamma);
 			}
@@ -405,9 +510,9 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 	double r2 = 0.0;
 	if (e < m.nElem) {
 		double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-		const uint16_t *cs = p.csr + (size_t) e * p.csrStride;
+		const uint16_t *cl = p.csr + (size_t) e * p.csrStride;
 		for (int i = 0; i < p.csrStride; i++) {
-			const unsigned code = cs[i];
+			const unsigned code = ldStream<NT>(cl + i);
 			if (code == 0xFFFFu) break;
 			const int j = (int) (code & 0x7FFFu);
 			if (code & 0x8000u) {
@@ -418,7 +523,13 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 				for (int k = 0; k < 5; k++) r[k] += lds[k * ms + j];
 			}
 		}
-		updateCell(m, in, out, q, W0, alpha, firstStage, e, r);
+		const int t = threadIdx.x;
+		if constexpr (LDSSTATE)
+			updateCell<NT>(m, out, q, W0, alpha, firstStage, e, r, cs[t], cs[kBlock + t], cs[2 * kBlock + t],
+			               cs[3 * kBlock + t], cs[4 * kBlock + t], cs[5 * kBlock + t]);
+		else
+			updateCell<NT>(m, out, q, W0, alpha, firstStage, e, r, in.rho[e], in.u[e], in.v[e], in.w[e], in.p[e],
+			               firstStage ? in.E[e] : 0.0);
 		r2 = r[0] * r[0];
 	}
 	if (rmsPartial) {
@@ -509,15 +620,26 @@ void launchStage(const DevMesh &m, DevState s, const DevFree &q, const double *f
 
 void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const double *Mint,
                       const DevFree &q, int scheme, double *W0, double alpha, int firstStage, int machDerived,
-                      double *rmsPartial, hipStream_t st) {
+                      double *rmsPartial, const StageTuning &t, hipStream_t st) {
 	if (p.nBlocks == 0) return;
-	size_t lds = sizeof(double) * 5 * (size_t) (p.maxSlots > kBlock ? p.maxSlots : kBlock);
-	if (scheme == 1)
-		hipLaunchKernelGGL(kStageFused<1>, dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, Mint, q, W0, alpha,
-		                   firstStage, machDerived, rmsPartial);
-	else
-		hipLaunchKernelGGL(kStageFused<0>, dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, Mint, q, W0, alpha,
-		                   firstStage, machDerived, rmsPartial);
+	size_t words = (size_t) 5 * p.maxSlots + (t.ldsState ? 7 * kBlock : 0);
+	size_t lds = sizeof(double) * (words > (size_t) kBlock ? words : (size_t) kBlock);
+	const int x = t.xcds;
+#define E3D_LAUNCH(S, L, N)                                                                                        \
+	hipLaunchKernelGGL((kStageFused<S, L, N>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, Mint, q, W0, \
+	                   alpha, firstStage, machDerived, rmsPartial, x)
+	const int code = (scheme == 1 ? 4 : 0) | (t.ldsState ? 2 : 0) | (t.nontemporal ? 1 : 0);
+	switch (code) {
+		case 0: E3D_LAUNCH(0, false, false); break;
+		case 1: E3D_LAUNCH(0, false, true); break;
+		case 2: E3D_LAUNCH(0, true, false); break;
+		case 3: E3D_LAUNCH(0, true, true); break;
+		case 4: E3D_LAUNCH(1, false, false); break;
+		case 5: E3D_LAUNCH(1, false, true); break;
+		case 6: E3D_LAUNCH(1, true, false); break;
+		default: E3D_LAUNCH(1, true, true); break;
+	}
+#undef E3D_LAUNCH
 }
 
 void launchSum(const double *in, int n, double *out, hipStream_t st) {
