@@ -101,8 +101,8 @@ struct e3d_ctx {
 	DBuf<int8_t> skind;
 	DBuf<double> snx, sny, snz, sS;
 	DBuf<uint16_t> csr;
-	// state: two buffers, st[cur] is the current one
-	DBuf<double> st[2][8]; // rho u v w p E H M
+	// state: two buffers of (nElem + nGhost) x 8 doubles, st[cur] is the current one
+	DBuf<double> st[2];
 	int cur = 0;
 	DBuf<double> W0, fs, partial, scalars;
 	// aero (wall list order)
@@ -111,10 +111,7 @@ struct e3d_ctx {
 	DBuf<double> wN[3], wC[3];
 	double cpFactor = 0.0;
 
-	e3d::DevState state(int which) {
-		auto &b = st[which];
-		return {b[0].p, b[1].p, b[2].p, b[3].p, b[4].p, b[5].p, b[6].p, b[7].p};
-	}
+	e3d::DevState state(int which) { return {st[which].p}; }
 	e3d::DevState state() { return state(cur); }
 	void sync() { hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
 	void check() { hipCheck(hipGetLastError(), "kernel launch"); }
@@ -142,10 +139,10 @@ void initFreestream(e3d_ctx *c) {
 	std::vector<double> h(c->nTot);
 	const double vals[8] = {c->fr.rho_inf, c->fr.u_inf, c->fr.v_inf, c->fr.w_inf, c->fr.p_inf, c->fr.E_inf, c->fr.H_inf, c->fr.M_inf};
 	c->cur = 0;
-	for (int k = 0; k < 8; k++) {
-		std::fill(h.begin(), h.end(), vals[k]);
-		c->st[0][k].upload(h.data(), c->nTot, c->stream);
-	}
+	h.resize((size_t) c->nTot * e3d::kStateWidth);
+	for (size_t i = 0; i < (size_t) c->nTot; i++)
+		for (int k = 0; k < 8; k++) h[i * e3d::kStateWidth + k] = vals[k];
+	c->st[0].upload(h.data(), h.size(), c->stream);
 	c->sync();
 	c->machDerived = 0;
 }
@@ -171,7 +168,7 @@ void runStage(e3d_ctx *c, int stage, double *rmsPartial) {
 	if (c->fused) {
 		auto o = c->state(c->cur ^ 1);
 		record(evA);
-		e3d::launchStageFused(c->dm, c->plan, s, o, s.M, c->fr, c->prm.scheme, c->W0.p, alphas[stage], stage == 0,
+		e3d::launchStageFused(c->dm, c->plan, s, o, c->fr, c->prm.scheme, c->W0.p, alphas[stage], stage == 0,
 		                      c->machDerived, rmsPartial, c->tune, c->stream);
 		record(evB);
 		c->cur ^= 1;
@@ -187,15 +184,15 @@ void runStage(e3d_ctx *c, int stage, double *rmsPartial) {
 
 const char *stageKernelName(const e3d_ctx *c) { return c->fused ? "kStageFused" : "kBc+kFlux+kStage"; }
 
-// host reference-order arrays <-> device-order arrays (interior permuted, ghosts as they are)
-void toDevice(const e3d_ctx *c, const double *ref, std::vector<double> &dev) {
-	dev.resize(c->nTot);
-	for (int i = 0; i < c->nElem; i++) dev[i] = ref[c->new2old[i]];
-	for (int i = c->nElem; i < c->nTot; i++) dev[i] = ref[i];
+// host reference-order field k <-> device state (interior permuted, ghosts as they are, 8 doubles
+// per cell)
+void toDevice(const e3d_ctx *c, const double *ref, int k, std::vector<double> &dev) {
+	for (int i = 0; i < c->nElem; i++) dev[(size_t) i * e3d::kStateWidth + k] = ref[c->new2old[i]];
+	for (int i = c->nElem; i < c->nTot; i++) dev[(size_t) i * e3d::kStateWidth + k] = ref[i];
 }
-void toReference(const e3d_ctx *c, const std::vector<double> &dev, double *ref) {
-	for (int i = 0; i < c->nElem; i++) ref[c->new2old[i]] = dev[i];
-	for (int i = c->nElem; i < c->nTot; i++) ref[i] = dev[i];
+void toReference(const e3d_ctx *c, const std::vector<double> &dev, int k, double *ref) {
+	for (int i = 0; i < c->nElem; i++) ref[c->new2old[i]] = dev[(size_t) i * e3d::kStateWidth + k];
+	for (int i = c->nElem; i < c->nTot; i++) ref[i] = dev[(size_t) i * e3d::kStateWidth + k];
 }
 
 // one iteration of Run's body; the sum of R_rho^2 of the last stage lands in *dsum (device)
@@ -456,7 +453,6 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		int order = (orderEnv && std::string(orderEnv) == "reference") ? e3d::ORDER_REFERENCE : e3d::ORDER_MORTON;
 		c->fused = !(pathEnv && std::string(pathEnv) == "unfused");
 		// launch-shape overrides for measurement (results are identical for every choice)
-		if (const char *v = std::getenv("E3D_LDS_STATE")) c->tune.ldsState = std::atoi(v) != 0;
 		if (const char *v = std::getenv("E3D_NONTEMPORAL")) c->tune.nontemporal = std::atoi(v) != 0;
 		if (const char *v = std::getenv("E3D_XCDS")) c->tune.xcds = std::max(1, std::atoi(v));
 		e3d::HostPlan P = e3d::buildPlan(sm, order, e3d::kCellsPerBlock);
@@ -505,8 +501,7 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		double uinf2 = c->fr.u_inf * c->fr.u_inf + c->fr.v_inf * c->fr.v_inf + c->fr.w_inf * c->fr.w_inf;
 		c->cpFactor = 1 / (0.5 * c->fr.rho_inf * uinf2);
 
-		for (auto &buf : c->st)
-			for (auto &b : buf) b.alloc(c->nTot);
+		for (auto &b : c->st) b.alloc((size_t) c->nTot * e3d::kStateWidth);
 		c->W0.alloc(5 * nE);
 		c->fs.alloc(5 * nF);
 		c->partial.alloc(std::max(1, e3d::stageBlocks(c->nElem)));
@@ -531,12 +526,10 @@ int e3d_set_state(e3d_ctx *c, const double *const fields[8]) {
 		if (!fields[k]) return fail(E3D_ERR_ARG, "null state field");
 	return guarded([&] {
 		hipCheck(hipSetDevice(c->device), "hipSetDevice");
-		std::vector<double> h;
-		for (int k = 0; k < 8; k++) {
-			toDevice(c, fields[k], h);
-			c->st[c->cur][k].upload(h.data(), c->nTot, c->stream);
-			c->sync();
-		}
+		std::vector<double> h((size_t) c->nTot * e3d::kStateWidth);
+		for (int k = 0; k < 8; k++) toDevice(c, fields[k], k, h);
+		c->st[c->cur].upload(h.data(), h.size(), c->stream);
+		c->sync();
 		c->machDerived = 0;
 		return E3D_OK;
 	});
@@ -550,15 +543,12 @@ int e3d_get_state(e3d_ctx *c, double *const fields[8]) {
 			e3d::launchMach(c->dm, c->state(), c->fr, c->stream);
 			c->check();
 		}
-		std::vector<double> h(c->nTot);
+		std::vector<double> h((size_t) c->nTot * e3d::kStateWidth);
+		hipCheck(hipMemcpyAsync(h.data(), c->st[c->cur].p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream),
+		         "hipMemcpyAsync D2H");
+		c->sync();
 		for (int k = 0; k < 8; k++)
-			if (fields[k]) {
-				hipCheck(hipMemcpyAsync(h.data(), c->st[c->cur][k].p, sizeof(double) * c->nTot, hipMemcpyDeviceToHost,
-				                        c->stream),
-				         "hipMemcpyAsync D2H");
-				c->sync();
-				toReference(c, h, fields[k]);
-			}
+			if (fields[k]) toReference(c, h, k, fields[k]);
 		return E3D_OK;
 	});
 }
@@ -668,7 +658,7 @@ int e3d_aero(e3d_ctx *c, double cf[3], double cm[3]) {
 		double h[6] = {0, 0, 0, 0, 0, 0};
 		if (c->nWall) {
 			e3d::launchAero(c->nWall, c->wallCell.p, c->wN[0].p, c->wN[1].p, c->wN[2].p, c->wC[0].p, c->wC[1].p, c->wC[2].p,
-			                c->st[c->cur][4].p, c->cpFactor, c->fr.p_inf, c->prm.cref, c->prm.sref, c->scalars.p, c->stream);
+			                c->state(), c->cpFactor, c->fr.p_inf, c->prm.cref, c->prm.sref, c->scalars.p, c->stream);
 			c->check();
 			hipCheck(hipMemcpyAsync(h, c->scalars.p, sizeof(h), hipMemcpyDeviceToHost, c->stream), "D2H");
 			c->sync();
@@ -783,8 +773,7 @@ void e3d_destroy(e3d_ctx *c) {
 	if (!c) return;
 	(void) hipSetDevice(c->device);
 	if (c->stream) (void) hipStreamSynchronize(c->stream);
-	for (auto &buf : c->st)
-		for (auto &b : buf) b.free();
+	for (auto &b : c->st) b.free();
 	c->W0.free(), c->fs.free(), c->partial.free(), c->scalars.free();
 	if (c->ev0) (void) hipEventDestroy(c->ev0);
 	if (c->ev1) (void) hipEventDestroy(c->ev1);

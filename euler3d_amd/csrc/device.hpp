@@ -11,8 +11,9 @@
 //                    nz, area S (reference face order; used by the per-face parity entry points)
 //   ghosts [nGhost]  kind (int8: 0 farfield, 1 wall, 2 symmetry); ghost k sits on face nFaceInt + k
 //   cells  [nElem]   volume, sum|Nx|, sum|Ny|, sum|Nz| (static parts of the local time step)
-//   state  2 x [nElem + nGhost] rho, u, v, w, p, E, H, M: ping-pong buffers, a stage reads one and
-//                    writes the other (the ghost part carries the farfield ghosts' previous state)
+//   state  2 x [nElem + nGhost][8] (rho, u, v, w, p, E, H, M) per cell, 64 B: ping-pong buffers, a
+//                    stage reads one and writes the other (the ghost part carries the farfield
+//                    ghosts' previous state)
 //   W0     [5][nElem] conservative state at the start of the iteration
 //   plan   per block of kCellsPerBlock cells: its faces (every face touching the block, faces cut
 //                    by the block boundary appear in both blocks) as slot records e0, e1, kind,
@@ -39,8 +40,11 @@ struct DevCells {
 	const int *flist;  // face id, bit 31 set when the cell is the face's elem1 (subtract)
 };
 
+// cell state, array of structures: kStateWidth doubles per cell in the order of StateField
+constexpr int kStateWidth = 8;
+enum StateField : int { S_RHO = 0, S_U, S_V, S_W, S_P, S_E, S_H, S_M };
 struct DevState {
-	double *rho, *u, *v, *w, *p, *E, *H, *M;
+	double *a; // (nElem + nGhost) * kStateWidth, 64 B aligned per cell
 };
 
 struct DevFree {
@@ -81,23 +85,23 @@ void launchStage(const DevMesh &m, DevState s, const DevFree &q, const double *f
 int stageBlocks(int nElem);
 // launch choices of the fused stage (they change speed only, never results)
 struct StageTuning {
-	bool ldsState = true;    // stage the block's own cells in LDS
 	bool nontemporal = true; // non-temporal loads for arrays read once per stage
 	int xcds = 8;            // XCD count for the L2-aware block order (1: plain dispatch order)
 };
 
 // Fused RK stage: boundary ghosts, face fluxes (once per block, in LDS), residual gather, local
-// dt and update, reading `in` and writing `out` (interior and ghosts).
-void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const double *Mint,
-                      const DevFree &q, int scheme, double *W0, double alpha, int firstStage, int machDerived,
-                      double *rmsPartial, const StageTuning &t, hipStream_t st);
+// dt and update, reading `in` and writing `out` (interior and ghosts). Before the first update
+// after a state upload (machDerived == 0) the interior Mach number comes from the state's M slot.
+void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const DevFree &q, int scheme,
+                      double *W0, double alpha, int firstStage, int machDerived, double *rmsPartial,
+                      const StageTuning &t, hipStream_t st);
 // deterministic sum of n doubles into out[0] (single workgroup, fixed order)
 void launchSum(const double *in, int n, double *out, hipStream_t st);
 // interior Mach number from the primitives (FlowField.cpp:157), for state read-back
 void launchMach(const DevMesh &m, DevState s, const DevFree &q, hipStream_t st);
 // wall-face force/moment: out[6] = (Fx,Fy,Fz, CPx,CPy,CPz) summed in a fixed order
 void launchAero(int nWall, const int *wallCell, const double *Nx, const double *Ny, const double *Nz,
-                const double *Cx, const double *Cy, const double *Cz, const double *p, double cpFactor, double pInf,
+                const double *Cx, const double *Cy, const double *Cz, DevState s, double cpFactor, double pInf,
                 double cref, double sref, double *out6, hipStream_t st);
 
 }// namespace e3d

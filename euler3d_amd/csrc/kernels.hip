@@ -7,6 +7,10 @@
 // ascending reference face id, the reference's accumulation order, so the result is
 // deterministic and bit-identical to the sequential face loop without atomics.
 //
+// Cell state is array-of-structures, 8 doubles = 64 B per cell (rho, u, v, w, p, E, H, M): a
+// face reading a neighbour cell touches one 64 B chunk with four 16 B loads instead of six
+// scattered cache lines.
+//
 // Two forms of an RK stage exist:
 //   kStageFused  the production path: one workgroup per block of kCellsPerBlock consecutive cells
 //                computes every face flux touching the block once into LDS (boundary ghosts
@@ -21,9 +25,6 @@ namespace e3d {
 namespace {
 
 constexpr int kBlock = 256;
-#ifndef E3D_FUSED_WAVES
-#define E3D_FUSED_WAVES 1
-#endif
 static_assert(kBlock == kCellsPerBlock, "one thread per cell in the fused stage");
 
 struct Prim {
@@ -42,19 +43,20 @@ __device__ __forceinline__ T ldStream(const T *p) {
 	else return *p;
 }
 
+__device__ __forceinline__ double *cellPtr(const DevState &s, int i) { return s.a + (size_t) i * kStateWidth; }
+
 __device__ __forceinline__ Prim loadPrim(const DevState &s, int i) {
-	return {s.rho[i], s.u[i], s.v[i], s.w[i], s.p[i], s.H[i]};
+	const double2 *c = reinterpret_cast<const double2 *>(cellPtr(s, i));
+	const double2 a = c[0], b = c[1], d = c[2], h = c[3]; // (rho,u) (v,w) (p,E) (H,M)
+	return {a.x, a.y, b.x, b.y, d.x, h.x};
 }
 
 __device__ __forceinline__ void storeGhost(const DevState &s, int g, const Ghost &G) {
-	s.rho[g] = G.rho;
-	s.u[g] = G.u;
-	s.v[g] = G.v;
-	s.w[g] = G.w;
-	s.p[g] = G.p;
-	s.E[g] = G.E;
-	s.H[g] = G.H;
-	s.M[g] = G.M;
+	double2 *c = reinterpret_cast<double2 *>(cellPtr(s, g));
+	c[0] = make_double2(G.rho, G.u);
+	c[1] = make_double2(G.v, G.w);
+	c[2] = make_double2(G.p, G.E);
+	c[3] = make_double2(G.H, G.M);
 }
 
 // Solver::Roe, ConvectiveFluxesSchemes.cpp:10-165
@@ -103,8 +105,7 @@ __device__ __forceinline__ void roeFlux(const Prim &L, const Prim &R, double nx,
 }
 
 // The same Roe flux, component by component: each of the five values is formed with exactly the
-// operations of roeFlux and handed to emit(k, F_k) at once, so no 5-vector of temporaries is live
-// (fewer VGPRs, more resident waves in the fused stage).
+// operations of roeFlux and handed to emit(k, F_k) at once, so no 5-vector of temporaries is live.
 template<typename Emit>
 __device__ __forceinline__ void roeFluxEmit(const Prim &L, const Prim &R, double nx, double ny, double nz, double gamma,
                                             Emit emit) {
@@ -206,8 +207,9 @@ __device__ __forceinline__ Ghost mirrorGhost(double ri, double ui, double vi, do
 __device__ __forceinline__ Ghost farfieldGhost(const DevState &in, int g, double Mi, double nx, double ny, double nz,
                                                const DevFree &q, const Prim &I, double Ei) {
 	double gm = q.gamma;
-	double Vg = in.u[g] * nx + in.v[g] * ny + in.w[g] * nz;
-	double Mg = in.M[g];
+	const double *gc = cellPtr(in, g);
+	double Vg = gc[S_U] * nx + gc[S_V] * ny + gc[S_W] * nz;
+	double Mg = gc[S_M];
 	double ri = I.rho, pi = I.p, ui = I.u, vi = I.v, wi = I.w;
 	if (Mg >= 1.0 && Vg > 0.0) return {ri, ui, vi, wi, pi, Ei, I.H, Mi};
 	if (Mg >= 1.0 && Vg <= 0.0) return {q.rho_inf, q.u_inf, q.v_inf, q.w_inf, q.p_inf, q.E_inf, q.H_inf, q.M_inf};
@@ -242,12 +244,14 @@ __global__ void __launch_bounds__(kBlock) kBc(DevMesh m, DevState s, DevFree q, 
 	int g = m.nElem + k;
 	int i = m.f.e0[f];
 	double nx = m.f.nx[f], ny = m.f.ny[f], nz = m.f.nz[f];
+	const double *ic = cellPtr(s, i);
+	Prim I = loadPrim(s, i);
 	Ghost G;
 	if (m.gkind[k] == 0) {
-		double Mi = machDerived ? machOf(s.rho[i], s.u[i], s.v[i], s.w[i], s.p[i], q.gamma) : s.M[i];
-		G = farfieldGhost(s, g, Mi, nx, ny, nz, q, loadPrim(s, i), s.E[i]);
+		double Mi = machDerived ? machOf(I.rho, I.u, I.v, I.w, I.p, q.gamma) : ic[S_M];
+		G = farfieldGhost(s, g, Mi, nx, ny, nz, q, I, ic[S_E]);
 	} else {
-		G = mirrorGhost(s.rho[i], s.u[i], s.v[i], s.w[i], s.p[i], nx, ny, nz, q.gamma);
+		G = mirrorGhost(I.rho, I.u, I.v, I.w, I.p, nx, ny, nz, q.gamma);
 	}
 	storeGhost(s, g, G);
 }
@@ -260,26 +264,25 @@ __global__ void __launch_bounds__(kBlock) kFlux(DevMesh m, DevState s, DevFree q
 	double nx = m.f.nx[f], ny = m.f.ny[f], nz = m.f.nz[f];
 	double F[5];
 	int kind = f < m.nFaceInt ? 3 : (int) m.gkind[f - m.nFaceInt];
+	Prim R = loadPrim(s, b);
 	if (kind == 3 || kind == 2) {
-		Prim L = loadPrim(s, a), R = loadPrim(s, b);
+		Prim L = loadPrim(s, a);
 		if (scheme == 1) ausmFlux(L, R, nx, ny, nz, q.gamma, F);
 		else roeFlux(L, R, nx, ny, nz, q.gamma, F);
 	} else if (kind == 1) {
-		double pg = s.p[b];
 		F[0] = 0;
-		F[1] = nx * pg;
-		F[2] = ny * pg;
-		F[3] = nz * pg;
+		F[1] = nx * R.p;
+		F[2] = ny * R.p;
+		F[3] = nz * R.p;
 		F[4] = 0;
 	} else {
 		// Solver::Fc with V = ghost velocity . n (ConvectiveFluxesSchemes.cpp:167-187)
-		double rho = s.rho[b], u = s.u[b], v = s.v[b], w = s.w[b], H = s.H[b], p = s.p[b];
-		double V = u * nx + v * ny + w * nz;
-		F[0] = rho * V;
-		F[1] = rho * u * V + p * nx;
-		F[2] = rho * v * V + p * ny;
-		F[3] = rho * w * V + p * nz;
-		F[4] = rho * H * V;
+		double V = R.u * nx + R.v * ny + R.w * nz;
+		F[0] = R.rho * V;
+		F[1] = R.rho * R.u * V + R.p * nx;
+		F[2] = R.rho * R.v * V + R.p * ny;
+		F[3] = R.rho * R.w * V + R.p * nz;
+		F[4] = R.rho * R.H * V;
 	}
 	double S = m.f.S[f];
 	size_t nF = (size_t) m.nFace;
@@ -334,7 +337,8 @@ __device__ __forceinline__ double localDt(const DevMesh &m, const DevFree &q, do
 __global__ void __launch_bounds__(kBlock) kTimestep(DevMesh m, DevState s, DevFree q, double *dt) {
 	int e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= m.nElem) return;
-	dt[e] = localDt(m, q, s.rho[e], s.u[e], s.v[e], s.w[e], s.p[e], e);
+	Prim P = loadPrim(s, e);
+	dt[e] = localDt(m, q, P.rho, P.u, P.v, P.w, P.p, e);
 }
 
 // deterministic block reduction of one double per thread (fixed tree)
@@ -349,7 +353,8 @@ __device__ __forceinline__ double blockSum(double x, double *lds) {
 }
 
 // RK stage update of one cell from its residual r: EulerSolver.cpp:431-449 / 459-470 and
-// FlowField.cpp:140-158. (rho, u, v, w, p, E) is the cell's state before the stage; writes `out`.
+// FlowField.cpp:140-158. (rho, u, v, w, p, E) is the cell's state before the stage; writes `out`
+// (the Mach slot is not written: it is recomputed from the primitives when needed).
 template<bool NT = false>
 __device__ __forceinline__ void updateCell(const DevMesh &m, const DevState &out, const DevFree &q, double *W0,
                                            double alpha, int firstStage, int e, const double r[5], double rho, double u,
@@ -379,13 +384,11 @@ __device__ __forceinline__ void updateCell(const DevMesh &m, const DevState &out
 	double g = q.gamma;
 	double np = (g - 1) * nr * (nE - (nu * nu + nv * nv + nw * nw) * 0.5);
 	double nH = nE + (np / nr);
-	out.rho[e] = nr;
-	out.u[e] = nu;
-	out.v[e] = nv;
-	out.w[e] = nw;
-	out.E[e] = nE;
-	out.p[e] = np;
-	out.H[e] = nH;
+	double2 *c = reinterpret_cast<double2 *>(cellPtr(out, e));
+	c[0] = make_double2(nr, nu);
+	c[1] = make_double2(nv, nw);
+	c[2] = make_double2(np, nE);
+	c[3].x = nH;
 }
 
 __global__ void __launch_bounds__(kBlock) kStage(DevMesh m, DevState s, DevFree q, const double *fs, double *W0, double alpha,
@@ -401,7 +404,8 @@ __global__ void __launch_bounds__(kBlock) kStage(DevMesh m, DevState s, DevFree 
 #pragma unroll
 			for (int k = 0; k < 5; k++) Rout[k * n + e] = r[k];
 		}
-		updateCell(m, s, q, W0, alpha, firstStage, e, r, s.rho[e], s.u[e], s.v[e], s.w[e], s.p[e], s.E[e]);
+		const double *c = cellPtr(s, e);
+		updateCell(m, s, q, W0, alpha, firstStage, e, r, c[S_RHO], c[S_U], c[S_V], c[S_W], c[S_P], c[S_E]);
 		r2 = r[0] * r[0];
 	}
 	if (rmsPartial) {
@@ -420,42 +424,15 @@ __device__ __forceinline__ int logicalBlock(int w, int nBlocks, int xcds) {
 	return x * q + min(x, r) + k;
 }
 
-// Fused RK stage over one block of cells (see the file comment). LDSSTATE stages the block's own
-// cells (rho, u, v, w, p, E, H) in LDS first; faces and the update then read them from LDS and
-// only neighbours across the block boundary come from global memory.
-template<int SCHEME, bool LDSSTATE, bool NT>
-__global__ void __launch_bounds__(kBlock, E3D_FUSED_WAVES) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out, const double *Mint,
-                                                      DevFree q, double *W0, double alpha, int firstStage, int machDerived,
+// Fused RK stage over one block of cells (see the file comment).
+template<int SCHEME, bool NT>
+__global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out, DevFree q,
+                                                      double *W0, double alpha, int firstStage, int machDerived,
                                                       double *rmsPartial, int xcds) {
 	extern __shared__ double lds[];
 	const int b = logicalBlock(blockIdx.x, p.nBlocks, xcds);
 	const int s0 = p.blockSlot[b], ns = p.blockSlot[b + 1] - s0;
 	const int ms = p.maxSlots;
-	const int c0 = b * kBlock;
-	const int nc = min(kBlock, m.nElem - c0);
-	double *cs = lds + 5 * ms; // [7][kBlock] own-cell state when LDSSTATE
-	if constexpr (LDSSTATE) {
-		const int t = threadIdx.x;
-		if (t < nc) {
-			const int e = c0 + t;
-			cs[0 * kBlock + t] = in.rho[e];
-			cs[1 * kBlock + t] = in.u[e];
-			cs[2 * kBlock + t] = in.v[e];
-			cs[3 * kBlock + t] = in.w[e];
-			cs[4 * kBlock + t] = in.p[e];
-			cs[5 * kBlock + t] = in.E[e];
-			cs[6 * kBlock + t] = in.H[e];
-		}
-		__syncthreads();
-	}
-	auto prim = [&](int i) -> Prim {
-		if constexpr (LDSSTATE) {
-			const unsigned l = (unsigned) (i - c0);
-			if (l < (unsigned) nc)
-				return {cs[l], cs[kBlock + l], cs[2 * kBlock + l], cs[3 * kBlock + l], cs[4 * kBlock + l], cs[6 * kBlock + l]};
-		}
-		return loadPrim(in, i);
-	};
 	// ---- face fluxes of the block, once each, into LDS
 	for (int j = threadIdx.x; j < ns; j += kBlock) {
 		const int s = s0 + j;
@@ -463,23 +440,21 @@ __global__ void __launch_bounds__(kBlock, E3D_FUSED_WAVES) kStageFused(DevMesh m
 		const int kind = ldStream<NT>(p.kind + s);
 		const double nx = ldStream<NT>(p.nx + s), ny = ldStream<NT>(p.ny + s), nz = ldStream<NT>(p.nz + s);
 		const double S = ldStream<NT>(p.S + s);
-		const Prim L = prim(a);
+		const Prim L = loadPrim(in, a);
 		auto emit = [&](int k, double f) { lds[k * ms + j] = f * S; };
 		if (SCHEME == 0 && kind == 0) {
-			roeFluxEmit(L, prim(c), nx, ny, nz, q.gamma, emit);
+			roeFluxEmit(L, loadPrim(in, c), nx, ny, nz, q.gamma, emit);
 			continue;
 		}
 		double F[5];
 		if (kind == 0) {
-			schemeFlux<SCHEME>(L, prim(c), nx, ny, nz, q.gamma, F);
+			schemeFlux<SCHEME>(L, loadPrim(in, c), nx, ny, nz, q.gamma, F);
 		} else {
+			const double *ac = cellPtr(in, a);
 			Ghost G;
 			if (kind == 3) {
-				double Mi = machDerived ? machOf(L.rho, L.u, L.v, L.w, L.p, q.gamma) : Mint[a];
-				double Ea;
-				if constexpr (LDSSTATE) Ea = cs[5 * kBlock + (a - c0)];
-				else Ea = in.E[a];
-				G = farfieldGhost(in, c, Mi, nx, ny, nz, q, L, Ea);
+				double Mi = machDerived ? machOf(L.rho, L.u, L.v, L.w, L.p, q.gamma) : ac[S_M];
+				G = farfieldGhost(in, c, Mi, nx, ny, nz, q, L, ac[S_E]);
 			} else {
 				G = mirrorGhost(L.rho, L.u, L.v, L.w, L.p, nx, ny, nz, q.gamma);
 			}
@@ -502,7 +477,7 @@ __global__ void __launch_bounds__(kBlock, E3D_FUSED_WAVES) kStageFused(DevMesh m
 			}
 		}
 #pragma unroll
-		for (int k = 0; k < 5; k++) lds[k * ms + j] = F[k] * S;
+		for (int k = 0; k < 5; k++) emit(k, F[k]);
 	}
 	__syncthreads();
 	// ---- one thread per cell: gather in reference order, local dt, update
@@ -523,13 +498,9 @@ __global__ void __launch_bounds__(kBlock, E3D_FUSED_WAVES) kStageFused(DevMesh m
 				for (int k = 0; k < 5; k++) r[k] += lds[k * ms + j];
 			}
 		}
-		const int t = threadIdx.x;
-		if constexpr (LDSSTATE)
-			updateCell<NT>(m, out, q, W0, alpha, firstStage, e, r, cs[t], cs[kBlock + t], cs[2 * kBlock + t],
-			               cs[3 * kBlock + t], cs[4 * kBlock + t], cs[5 * kBlock + t]);
-		else
-			updateCell<NT>(m, out, q, W0, alpha, firstStage, e, r, in.rho[e], in.u[e], in.v[e], in.w[e], in.p[e],
-			               firstStage ? in.E[e] : 0.0);
+		const double2 *cc = reinterpret_cast<const double2 *>(cellPtr(in, e));
+		const double2 ru = cc[0], vw = cc[1], pE = cc[2];
+		updateCell<NT>(m, out, q, W0, alpha, firstStage, e, r, ru.x, ru.y, vw.x, vw.y, pE.x, pE.y);
 		r2 = r[0] * r[0];
 	}
 	if (rmsPartial) {
@@ -539,10 +510,11 @@ __global__ void __launch_bounds__(kBlock, E3D_FUSED_WAVES) kStageFused(DevMesh m
 	}
 }
 
+// deterministic sum of n doubles: each of kBlock threads adds a contiguous chunk in index order,
+// then a fixed tree
 __global__ void __launch_bounds__(kBlock) kSum(const double *in, int n, double *out) {
 	__shared__ double lds[kBlock];
 	double acc = 0.0;
-	// contiguous chunk per thread, summed in index order
 	int per = (n + kBlock - 1) / kBlock;
 	int b = threadIdx.x * per, en = min(n, b + per);
 	for (int i = b; i < en; i++) acc += in[i];
@@ -553,21 +525,22 @@ __global__ void __launch_bounds__(kBlock) kSum(const double *in, int n, double *
 __global__ void __launch_bounds__(kBlock) kMach(DevMesh m, DevState s, DevFree q) {
 	int e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= m.nElem) return;
-	s.M[e] = machOf(s.rho[e], s.u[e], s.v[e], s.w[e], s.p[e], q.gamma);
+	double *c = cellPtr(s, e);
+	c[S_M] = machOf(c[S_RHO], c[S_U], c[S_V], c[S_W], c[S_P], q.gamma);
 }
 
 // AeroCoefficients::Update (AeroCoefficients.cpp:38-52): per wall face coeff = N*cp*Cref/Sref and
 // coeff*x_face; summed per thread over a contiguous chunk, then a fixed tree
 __global__ void __launch_bounds__(kBlock) kAero(int nWall, const int *wallCell, const double *Nx, const double *Ny,
                                                 const double *Nz, const double *Cx, const double *Cy, const double *Cz,
-                                                const double *p, double cpFactor, double pInf, double cref, double sref,
+                                                DevState s, double cpFactor, double pInf, double cref, double sref,
                                                 double *out6) {
 	__shared__ double lds[kBlock];
 	double acc[6] = {0, 0, 0, 0, 0, 0};
 	int per = (nWall + kBlock - 1) / kBlock;
 	int b = threadIdx.x * per, en = min(nWall, b + per);
 	for (int i = b; i < en; i++) {
-		double cp = cpFactor * (p[wallCell[i]] - pInf);
+		double cp = cpFactor * (cellPtr(s, wallCell[i])[S_P] - pInf);
 		double cx = Nx[i] * cp * cref / sref;
 		double cy = Ny[i] * cp * cref / sref;
 		double cz = Nz[i] * cp * cref / sref;
@@ -618,26 +591,22 @@ void launchStage(const DevMesh &m, DevState s, const DevFree &q, const double *f
 	                   rmsPartial, Rout);
 }
 
-void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const double *Mint,
-                      const DevFree &q, int scheme, double *W0, double alpha, int firstStage, int machDerived,
-                      double *rmsPartial, const StageTuning &t, hipStream_t st) {
+void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const DevFree &q, int scheme,
+                      double *W0, double alpha, int firstStage, int machDerived, double *rmsPartial,
+                      const StageTuning &t, hipStream_t st) {
 	if (p.nBlocks == 0) return;
-	size_t words = (size_t) 5 * p.maxSlots + (t.ldsState ? 7 * kBlock : 0);
+	size_t words = (size_t) 5 * p.maxSlots;
 	size_t lds = sizeof(double) * (words > (size_t) kBlock ? words : (size_t) kBlock);
 	const int x = t.xcds;
-#define E3D_LAUNCH(S, L, N)                                                                                        \
-	hipLaunchKernelGGL((kStageFused<S, L, N>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, Mint, q, W0, \
-	                   alpha, firstStage, machDerived, rmsPartial, x)
-	const int code = (scheme == 1 ? 4 : 0) | (t.ldsState ? 2 : 0) | (t.nontemporal ? 1 : 0);
-	switch (code) {
-		case 0: E3D_LAUNCH(0, false, false); break;
-		case 1: E3D_LAUNCH(0, false, true); break;
-		case 2: E3D_LAUNCH(0, true, false); break;
-		case 3: E3D_LAUNCH(0, true, true); break;
-		case 4: E3D_LAUNCH(1, false, false); break;
-		case 5: E3D_LAUNCH(1, false, true); break;
-		case 6: E3D_LAUNCH(1, true, false); break;
-		default: E3D_LAUNCH(1, true, true); break;
+#define E3D_LAUNCH(S, N)                                                                                          \
+	hipLaunchKernelGGL((kStageFused<S, N>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, q, W0, alpha, \
+	                   firstStage, machDerived, rmsPartial, x)
+	if (scheme == 1) {
+		if (t.nontemporal) E3D_LAUNCH(1, true);
+		else E3D_LAUNCH(1, false);
+	} else {
+		if (t.nontemporal) E3D_LAUNCH(0, true);
+		else E3D_LAUNCH(0, false);
 	}
 #undef E3D_LAUNCH
 }
@@ -652,9 +621,9 @@ void launchMach(const DevMesh &m, DevState s, const DevFree &q, hipStream_t st) 
 }
 
 void launchAero(int nWall, const int *wallCell, const double *Nx, const double *Ny, const double *Nz, const double *Cx,
-                const double *Cy, const double *Cz, const double *p, double cpFactor, double pInf, double cref,
-                double sref, double *out6, hipStream_t st) {
-	hipLaunchKernelGGL(kAero, dim3(1), dim3(kBlock), 0, st, nWall, wallCell, Nx, Ny, Nz, Cx, Cy, Cz, p, cpFactor, pInf,
+                const double *Cy, const double *Cz, DevState s, double cpFactor, double pInf, double cref, double sref,
+                double *out6, hipStream_t st) {
+	hipLaunchKernelGGL(kAero, dim3(1), dim3(kBlock), 0, st, nWall, wallCell, Nx, Ny, Nz, Cx, Cy, Cz, s, cpFactor, pInf,
 	                   cref, sref, out6);
 }
 
