@@ -101,6 +101,8 @@ struct e3d_ctx {
 	DBuf<int8_t> skind;
 	DBuf<double> snx, sny, snz, sS;
 	DBuf<uint16_t> csr;
+	DBuf<uint32_t> spack;
+	DBuf<int> blockHalo, haloCell;
 	// state: two buffers of (nElem + nGhost) x 8 doubles, st[cur] is the current one
 	DBuf<double> st[2];
 	int cur = 0;
@@ -456,7 +458,7 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		if (const char *v = std::getenv("E3D_NONTEMPORAL")) c->tune.nontemporal = std::atoi(v) != 0;
 		if (const char *v = std::getenv("E3D_XCDS")) c->tune.xcds = std::max(1, std::atoi(v));
 		e3d::HostPlan P = e3d::buildPlan(sm, order, e3d::kCellsPerBlock);
-		if ((size_t) P.maxSlots * 5 * sizeof(double) > 160 * 1024)
+		if (((size_t) P.maxSlots * 5 + (size_t) 6 * (e3d::kCellsPerBlock + P.maxHalo)) * sizeof(double) > 160 * 1024)
 			return fail(E3D_ERR_UNSUPPORTED, "a block of the fused stage needs more LDS than a CU has");
 		c->new2old = P.new2old;
 
@@ -486,8 +488,12 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		c->snz.alloc(nS), c->snz.upload(P.snz.data(), nS, s);
 		c->sS.alloc(nS), c->sS.upload(P.sS.data(), nS, s);
 		c->csr.alloc(P.csr.size()), c->csr.upload(P.csr.data(), P.csr.size(), s);
+		c->spack.alloc(nS), c->spack.upload(P.spack.data(), nS, s);
+		c->blockHalo.alloc(P.blockHalo.size()), c->blockHalo.upload(P.blockHalo.data(), P.blockHalo.size(), s);
+		c->haloCell.alloc(std::max<size_t>(1, P.haloCell.size())), c->haloCell.upload(P.haloCell.data(), P.haloCell.size(), s);
 		c->plan = {P.nBlocks, P.maxSlots, P.csrStride, c->blockSlot.p, c->se0.p, c->se1.p, c->skind.p,
-		           c->snx.p, c->sny.p, c->snz.p, c->sS.p, c->csr.p};
+		           c->snx.p, c->sny.p, c->snz.p, c->sS.p, c->csr.p, c->spack.p, c->blockHalo.p, c->haloCell.p,
+		           P.maxHalo};
 		// wall list in the reference's order for the force integration
 		c->nWall = (int) P.wallCell.size();
 		if (c->nWall) {

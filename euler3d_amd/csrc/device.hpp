@@ -69,6 +69,10 @@ struct DevPlan {
 	const int8_t *kind;      // per slot: 0 interior, 1 wall, 2 symmetry, 3 farfield
 	const double *nx, *ny, *nz, *S;
 	const uint16_t *csr;     // nElem * csrStride
+	const uint32_t *pack;    // per slot: local e0 | local e1 << 11 | kind << 22
+	const int *blockHalo;    // nBlocks + 1
+	const int *haloCell;     // outside cells read by each block, device ids
+	int maxHalo;
 };
 
 void launchBc(const DevMesh &m, DevState s, const DevFree &q, int machDerived, hipStream_t st);

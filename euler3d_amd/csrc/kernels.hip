@@ -424,35 +424,97 @@ __device__ __forceinline__ int logicalBlock(int w, int nBlocks, int xcds) {
 	return x * q + min(x, r) + k;
 }
 
-// Fused RK stage over one block of cells (see the file comment).
+// Fused RK stage over one block of cells (see the file comment). Every global load of the block
+// is issued before any flux is computed, so a block waits for memory once: the block's own cells
+// and its halo (the outside cells its faces read) are staged in LDS as rho, u, v, w, p, H; the
+// update's inputs (W0, V, sum|S|, face slots) are loaded into registers. The flux loop then reads
+// LDS only, except for boundary faces, whose ghost cells live in global memory.
 template<int SCHEME, bool NT>
 __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out, DevFree q,
                                                       double *W0, double alpha, int firstStage, int machDerived,
                                                       double *rmsPartial, int xcds) {
 	extern __shared__ double lds[];
 	const int b = logicalBlock(blockIdx.x, p.nBlocks, xcds);
+	const int t = threadIdx.x;
+	const int c0 = b * kBlock;
+	const int nc = min(kBlock, m.nElem - c0);
 	const int s0 = p.blockSlot[b], ns = p.blockSlot[b + 1] - s0;
+	const int h0 = p.blockHalo[b], nh = p.blockHalo[b + 1] - h0;
+	const int cap = kBlock + p.maxHalo;
 	const int ms = p.maxSlots;
+	double *ls = lds;           // [6][cap]: rho, u, v, w, p, H of own cells then halo cells
+	double *lf = lds + 6 * cap; // [5][ms]: F*S per slot
+	const size_t n = (size_t) m.nElem;
+
+	// ---- issue every load of the block
+	const int e = c0 + t;
+	const bool own = t < nc;
+	double2 ru = make_double2(0, 0), vw = ru, pE = ru, HM = ru;
+	if (own) {
+		const double2 *cc = reinterpret_cast<const double2 *>(cellPtr(in, e));
+		ru = cc[0], vw = cc[1], pE = cc[2], HM = cc[3];
+	}
+	double w0[5], vol = 0, sx = 0, sy = 0, sz = 0;
+	uint16_t code[6];
+	if (own) {
+		if (!firstStage) {
+#pragma unroll
+			for (int k = 0; k < 5; k++) w0[k] = ldStream<NT>(W0 + k * n + e);
+		}
+		vol = ldStream<NT>(m.c.vol + e);
+		sx = ldStream<NT>(m.c.sx + e);
+		sy = ldStream<NT>(m.c.sy + e);
+		sz = ldStream<NT>(m.c.sz + e);
+		const uint16_t *cl = p.csr + (size_t) e * p.csrStride;
+#pragma unroll
+		for (int i = 0; i < 6; i++) code[i] = i < p.csrStride ? ldStream<NT>(cl + i) : (uint16_t) 0xFFFF;
+	}
+	for (int h = t; h < nh; h += kBlock) {
+		const double2 *hc = reinterpret_cast<const double2 *>(cellPtr(in, p.haloCell[h0 + h]));
+		const double2 a0 = hc[0], a1 = hc[1], a2 = hc[2], a3 = hc[3];
+		const int l = kBlock + h;
+		ls[0 * cap + l] = a0.x;
+		ls[1 * cap + l] = a0.y;
+		ls[2 * cap + l] = a1.x;
+		ls[3 * cap + l] = a1.y;
+		ls[4 * cap + l] = a2.x;
+		ls[5 * cap + l] = a3.x;
+	}
+	if (own) {
+		ls[0 * cap + t] = ru.x;
+		ls[1 * cap + t] = ru.y;
+		ls[2 * cap + t] = vw.x;
+		ls[3 * cap + t] = vw.y;
+		ls[4 * cap + t] = pE.x;
+		ls[5 * cap + t] = HM.x;
+	}
+	__syncthreads();
+
 	// ---- face fluxes of the block, once each, into LDS
-	for (int j = threadIdx.x; j < ns; j += kBlock) {
+	auto lprim = [&](int l) -> Prim {
+		return {ls[l], ls[cap + l], ls[2 * cap + l], ls[3 * cap + l], ls[4 * cap + l], ls[5 * cap + l]};
+	};
+	for (int j = t; j < ns; j += kBlock) {
 		const int s = s0 + j;
-		const int a = ldStream<NT>(p.e0 + s), c = ldStream<NT>(p.e1 + s);
-		const int kind = ldStream<NT>(p.kind + s);
+		const uint32_t pk = ldStream<NT>(p.pack + s);
 		const double nx = ldStream<NT>(p.nx + s), ny = ldStream<NT>(p.ny + s), nz = ldStream<NT>(p.nz + s);
 		const double S = ldStream<NT>(p.S + s);
-		const Prim L = loadPrim(in, a);
-		auto emit = [&](int k, double f) { lds[k * ms + j] = f * S; };
+		const int l0 = (int) (pk & 0x7FFu), l1 = (int) ((pk >> 11) & 0x7FFu), kind = (int) (pk >> 22);
+		const Prim L = lprim(l0);
+		auto emit = [&](int k, double f) { lf[k * ms + j] = f * S; };
 		if (SCHEME == 0 && kind == 0) {
-			roeFluxEmit(L, loadPrim(in, c), nx, ny, nz, q.gamma, emit);
+			roeFluxEmit(L, lprim(l1), nx, ny, nz, q.gamma, emit);
 			continue;
 		}
 		double F[5];
 		if (kind == 0) {
-			schemeFlux<SCHEME>(L, loadPrim(in, c), nx, ny, nz, q.gamma, F);
+			schemeFlux<SCHEME>(L, lprim(l1), nx, ny, nz, q.gamma, F);
 		} else {
-			const double *ac = cellPtr(in, a);
+			const int a = c0 + l0;    // the interior cell of a boundary face is an own cell
+			const int c = p.e1[s];    // its ghost
 			Ghost G;
 			if (kind == 3) {
+				const double *ac = cellPtr(in, a);
 				double Mi = machDerived ? machOf(L.rho, L.u, L.v, L.w, L.p, q.gamma) : ac[S_M];
 				G = farfieldGhost(in, c, Mi, nx, ny, nz, q, L, ac[S_E]);
 			} else {
@@ -480,33 +542,62 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 		for (int k = 0; k < 5; k++) emit(k, F[k]);
 	}
 	__syncthreads();
-	// ---- one thread per cell: gather in reference order, local dt, update
-	const int e = b * kBlock + threadIdx.x;
+
+	// ---- one thread per cell: gather in reference order, local dt, RK update
 	double r2 = 0.0;
-	if (e < m.nElem) {
+	if (own) {
 		double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-		const uint16_t *cl = p.csr + (size_t) e * p.csrStride;
-		for (int i = 0; i < p.csrStride; i++) {
-			const unsigned code = ldStream<NT>(cl + i);
-			if (code == 0xFFFFu) break;
-			const int j = (int) (code & 0x7FFFu);
-			if (code & 0x8000u) {
 #pragma unroll
-				for (int k = 0; k < 5; k++) r[k] -= lds[k * ms + j];
+		for (int i = 0; i < 6; i++) {
+			const unsigned cd = code[i];
+			if (cd == 0xFFFFu) break;
+			const int j = (int) (cd & 0x7FFFu);
+			if (cd & 0x8000u) {
+#pragma unroll
+				for (int k = 0; k < 5; k++) r[k] -= lf[k * ms + j];
 			} else {
 #pragma unroll
-				for (int k = 0; k < 5; k++) r[k] += lds[k * ms + j];
+				for (int k = 0; k < 5; k++) r[k] += lf[k * ms + j];
 			}
 		}
-		const double2 *cc = reinterpret_cast<const double2 *>(cellPtr(in, e));
-		const double2 ru = cc[0], vw = cc[1], pE = cc[2];
-		updateCell<NT>(m, out, q, W0, alpha, firstStage, e, r, ru.x, ru.y, vw.x, vw.y, pE.x, pE.y);
+		const double rho = ru.x, u = ru.y, v = vw.x, w = vw.y, pp = pE.x;
+		if (firstStage) {
+			// W0 = (rho, u*rho, v*rho, w*rho, E*rho), EulerSolver.cpp:431-438
+			w0[0] = rho;
+			w0[1] = u * rho;
+			w0[2] = v * rho;
+			w0[3] = w * rho;
+			w0[4] = pE.y * rho;
+#pragma unroll
+			for (int k = 0; k < 5; k++) W0[k * n + e] = w0[k];
+		}
+		// TimeStepMethod2 (TimeIntegration.cpp:8-40) and FlowField::updateWRungeKutta
+		// (FlowField.cpp:140-158), as updateCell
+		double c = sqrt(q.gamma * pp / rho);
+		double lx = 0.5 * (fabs(u) + c) * sx;
+		double ly = 0.5 * (fabs(v) + c) * sy;
+		double lz = 0.5 * (fabs(w) + c) * sz;
+		double dt = q.cfl * vol / (lx + ly + lz);
+		double nr = w0[0] - alpha * dt * r[0] / vol;
+		double nru = w0[1] - alpha * dt * r[1] / vol;
+		double nrv = w0[2] - alpha * dt * r[2] / vol;
+		double nrw = w0[3] - alpha * dt * r[3] / vol;
+		double nrE = w0[4] - alpha * dt * r[4] / vol;
+		double nu = nru / nr, nv = nrv / nr, nw = nrw / nr, nE = nrE / nr;
+		double g = q.gamma;
+		double np = (g - 1) * nr * (nE - (nu * nu + nv * nv + nw * nw) * 0.5);
+		double nH = nE + (np / nr);
+		double2 *oc = reinterpret_cast<double2 *>(cellPtr(out, e));
+		oc[0] = make_double2(nr, nu);
+		oc[1] = make_double2(nv, nw);
+		oc[2] = make_double2(np, nE);
+		oc[3].x = nH;
 		r2 = r[0] * r[0];
 	}
 	if (rmsPartial) {
 		__syncthreads();
-		double t = blockSum(r2, lds);
-		if (threadIdx.x == 0) rmsPartial[b] = t;
+		double tt = blockSum(r2, lds);
+		if (t == 0) rmsPartial[b] = tt;
 	}
 }
 
@@ -595,7 +686,7 @@ void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState 
                       double *W0, double alpha, int firstStage, int machDerived, double *rmsPartial,
                       const StageTuning &t, hipStream_t st) {
 	if (p.nBlocks == 0) return;
-	size_t words = (size_t) 5 * p.maxSlots;
+	size_t words = (size_t) 6 * (kBlock + p.maxHalo) + (size_t) 5 * p.maxSlots;
 	size_t lds = sizeof(double) * (words > (size_t) kBlock ? words : (size_t) kBlock);
 	const int x = t.xcds;
 #define E3D_LAUNCH(S, N)                                                                                          \

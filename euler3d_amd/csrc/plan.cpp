@@ -170,6 +170,32 @@ HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock) {
 		P.maxSlots = std::max(P.maxSlots, ns);
 	}
 
+	// halo lists and block-local slot indices
+	P.blockHalo.assign(P.nBlocks + 1, 0);
+	P.spack.resize(P.se0.size());
+	std::vector<int> halo;
+	for (int blk = 0; blk < P.nBlocks; blk++) {
+		int c0 = blk * cellsPerBlock, c1 = std::min(nE, c0 + cellsPerBlock);
+		int s0 = P.blockSlot[blk], s1 = P.blockSlot[blk + 1];
+		halo.clear();
+		for (int s = s0; s < s1; s++)
+			for (int cell : {P.se0[s], P.se1[s]})
+				if (cell < nE && (cell < c0 || cell >= c1)) halo.push_back(cell);
+		std::sort(halo.begin(), halo.end());
+		halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+		if (cellsPerBlock + (int) halo.size() > 0x7FF) throw std::runtime_error("fused-stage block halo too large");
+		auto local = [&](int cell) -> uint32_t {
+			if (cell >= nE) return 0x7FF;
+			if (cell >= c0 && cell < c1) return (uint32_t) (cell - c0);
+			return (uint32_t) (cellsPerBlock + (std::lower_bound(halo.begin(), halo.end(), cell) - halo.begin()));
+		};
+		for (int s = s0; s < s1; s++)
+			P.spack[s] = local(P.se0[s]) | local(P.se1[s]) << 11 | (uint32_t) P.skind[s] << 22;
+		P.haloCell.insert(P.haloCell.end(), halo.begin(), halo.end());
+		P.blockHalo[blk + 1] = (int) P.haloCell.size();
+		P.maxHalo = std::max(P.maxHalo, (int) halo.size());
+	}
+
 	// wall list for the force integration, in the reference's order (Mesh.hpp:594-627)
 	size_t nw = sm.wallGhost.size();
 	P.wallCell.resize(nw);

@@ -29,6 +29,13 @@ struct HostPlan {
 	std::vector<int> blockSlot;
 	std::vector<int> se0, se1;
 	std::vector<int8_t> skind;
+	// block-local view of the slots: bits 0-10 local index of e0, 11-21 local index of e1, 22-23
+	// kind. Local index: own cell i -> i - c0; halo cell -> cellsPerBlock + its rank in the
+	// block's halo list; e1 of a boundary slot is a ghost and has no local index (0x7FF).
+	std::vector<uint32_t> spack;
+	std::vector<int> blockHalo;  // nBlocks + 1
+	std::vector<int> haloCell;   // per block: the interior cells outside it that its faces read, ascending
+	int maxHalo = 0;
 	std::vector<double> snx, sny, snz, sS;
 	std::vector<uint16_t> csr;
 
