@@ -488,6 +488,14 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 		ls[4 * cap + t] = pE.x;
 		ls[5 * cap + t] = HM.x;
 	}
+	// face slot record of this thread's first round (later rounds are prefetched one ahead)
+	uint32_t pkN = 0;
+	double nxN = 0, nyN = 0, nzN = 0, SN = 0;
+	if (t < ns) {
+		pkN = ldStream<NT>(p.pack + s0 + t);
+		nxN = ldStream<NT>(p.nx + s0 + t), nyN = ldStream<NT>(p.ny + s0 + t), nzN = ldStream<NT>(p.nz + s0 + t);
+		SN = ldStream<NT>(p.S + s0 + t);
+	}
 	__syncthreads();
 
 	// ---- face fluxes of the block, once each, into LDS
@@ -496,9 +504,13 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 	};
 	for (int j = t; j < ns; j += kBlock) {
 		const int s = s0 + j;
-		const uint32_t pk = ldStream<NT>(p.pack + s);
-		const double nx = ldStream<NT>(p.nx + s), ny = ldStream<NT>(p.ny + s), nz = ldStream<NT>(p.nz + s);
-		const double S = ldStream<NT>(p.S + s);
+		const uint32_t pk = pkN;
+		const double nx = nxN, ny = nyN, nz = nzN, S = SN;
+		if (j + kBlock < ns) {
+			pkN = ldStream<NT>(p.pack + s + kBlock);
+			nxN = ldStream<NT>(p.nx + s + kBlock), nyN = ldStream<NT>(p.ny + s + kBlock);
+			nzN = ldStream<NT>(p.nz + s + kBlock), SN = ldStream<NT>(p.S + s + kBlock);
+		}
 		const int l0 = (int) (pk & 0x7FFu), l1 = (int) ((pk >> 11) & 0x7FFu), kind = (int) (pk >> 22);
 		const Prim L = lprim(l0);
 		auto emit = [&](int k, double f) { lf[k * ms + j] = f * S; };
