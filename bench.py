@@ -38,6 +38,19 @@ def algorithmic_bytes_per_cell_stage(n_cells: int, n_faces: int) -> float:
     return 152.0 + 32.0 * n_faces / n_cells
 
 
+def measured_traffic(kernel: str, n_cells: int, scheme: int):
+    """HBM bytes per launch of the stage kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    runs (profiles/traffic.json), each counter scaled by what it reads for a plain copy of known
+    size; None when no profile of this kernel and workload is recorded."""
+    path = ROOT / "profiles" / "traffic.json"
+    if not path.exists():
+        return None
+    rec = json.loads(path.read_text()).get(f"{kernel}/{n_cells}/{'ausm' if scheme else 'roe'}")
+    if not rec:
+        return None
+    return rec["fetch_kib"] * rec["fetch_scale"] * 1024 + rec["write_kib"] * rec["write_scale"] * 1024
+
+
 def cpu_baseline(mesh, params: dict, n_cells: int) -> dict:
     sys.path.insert(0, str(ROOT / "oracle"))
     from oracle_ctypes import Oracle
@@ -127,7 +140,7 @@ def main() -> None:
                                f"{'AUSM' if a.scheme else 'Roe'}, RK5 CFL 1.5, FP64",
                    "cells": n_cells, "faces": n_faces, "stages_per_step": 5},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname, n_cells, a.scheme),
                      "kernel": kname, "bytes_per_cell_stage": bpc, "kernel_ms": kern_ms},
         "cpu_baseline": None,
     }

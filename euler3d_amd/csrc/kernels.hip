@@ -613,16 +613,20 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 	}
 }
 
-// deterministic sum of n doubles: each of kBlock threads adds a contiguous chunk in index order,
-// then a fixed tree
-__global__ void __launch_bounds__(kBlock) kSum(const double *in, int n, double *out) {
-	__shared__ double lds[kBlock];
+// deterministic sum of n doubles: each of kSumThreads threads adds the strided elements
+// i = t, t + kSumThreads, ... in index order (independent loads, coalesced), then a fixed tree
+constexpr int kSumThreads = 1024;
+__global__ void __launch_bounds__(kSumThreads) kSum(const double *in, int n, double *out) {
+	__shared__ double lds[kSumThreads];
 	double acc = 0.0;
-	int per = (n + kBlock - 1) / kBlock;
-	int b = threadIdx.x * per, en = min(n, b + per);
-	for (int i = b; i < en; i++) acc += in[i];
-	double t = blockSum(acc, lds);
-	if (threadIdx.x == 0) out[0] = t;
+	for (int i = threadIdx.x; i < n; i += kSumThreads) acc += in[i];
+	lds[threadIdx.x] = acc;
+	__syncthreads();
+	for (int off = kSumThreads / 2; off > 0; off >>= 1) {
+		if ((int) threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) out[0] = lds[0];
 }
 
 __global__ void __launch_bounds__(kBlock) kMach(DevMesh m, DevState s, DevFree q) {
@@ -715,7 +719,7 @@ void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState 
 }
 
 void launchSum(const double *in, int n, double *out, hipStream_t st) {
-	hipLaunchKernelGGL(kSum, dim3(1), dim3(kBlock), 0, st, in, n, out);
+	hipLaunchKernelGGL(kSum, dim3(1), dim3(kSumThreads), 0, st, in, n, out);
 }
 
 void launchMach(const DevMesh &m, DevState s, const DevFree &q, hipStream_t st) {
