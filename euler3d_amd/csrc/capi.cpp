@@ -457,6 +457,7 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		// launch-shape overrides for measurement (results are identical for every choice)
 		if (const char *v = std::getenv("E3D_NONTEMPORAL")) c->tune.nontemporal = std::atoi(v) != 0;
 		if (const char *v = std::getenv("E3D_XCDS")) c->tune.xcds = std::max(1, std::atoi(v));
+		if (const char *v = std::getenv("E3D_SHARED_RECIP")) c->tune.sharedRecip = std::atoi(v) != 0;
 		e3d::HostPlan P = e3d::buildPlan(sm, order, e3d::kCellsPerBlock);
 		if (((size_t) P.maxSlots * 5 + (size_t) 6 * (e3d::kCellsPerBlock + P.maxHalo)) * sizeof(double) > 160 * 1024)
 			return fail(E3D_ERR_UNSUPPORTED, "a block of the fused stage needs more LDS than a CU has");

@@ -91,6 +91,9 @@ int stageBlocks(int nElem);
 struct StageTuning {
 	bool nontemporal = true; // non-temporal loads for arrays read once per stage
 	int xcds = 8;            // XCD count for the L2-aware block order (1: plain dispatch order)
+	// quotients sharing a denominator use one IEEE reciprocal plus Markstein's FMA correction,
+	// which is correctly rounded and so bit-identical to the divisions it replaces
+	bool sharedRecip = true;
 };
 
 // Fused RK stage: boundary ghosts, face fluxes (once per block, in LDS), residual gather, local

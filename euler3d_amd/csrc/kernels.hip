@@ -104,17 +104,39 @@ __device__ __forceinline__ void roeFlux(const Prim &L, const Prim &R, double nx,
 	}
 }
 
+// Quotient x / d given y = RN(1 / d), correctly rounded (Markstein, IBM J. Res. Dev. 34 (1990)
+// 111-119): q = RN(x y) is within one ulp of x / d, the remainder x - q d is exact under an FMA,
+// and RN(q + (x - q d) y) is RN(x / d). Several quotients by one denominator then cost one IEEE
+// division plus a multiply and two FMAs each. A zero quotient is returned as q so that its sign
+// matches IEEE division. (Finite, normal operands; the solver's values are.)
+__device__ __forceinline__ double divRecip(double x, double d, double y) {
+	const double q = x * y;
+	const double e = fma(-q, d, x);
+	const double r = fma(e, y, q);
+	return q == 0.0 ? q : r;
+}
+
+// x / d either as the IEEE division or through the shared reciprocal y of d
+template<bool MK>
+__device__ __forceinline__ double quot(double x, double d, double y) {
+	if constexpr (MK) return divRecip(x, d, y);
+	else return x / d;
+}
+
 // The same Roe flux, component by component: each of the five values is formed with exactly the
 // operations of roeFlux and handed to emit(k, F_k) at once, so no 5-vector of temporaries is live.
-template<typename Emit>
+// With MK the quotients by sL + sR and by 2 c~^2 go through divRecip.
+template<bool MK = false, typename Emit>
 __device__ __forceinline__ void roeFluxEmit(const Prim &L, const Prim &R, double nx, double ny, double nz, double gamma,
                                             Emit emit) {
 	double sL = sqrt(L.rho), sR = sqrt(R.rho);
 	double rhoT = sqrt(L.rho * R.rho);
-	double uT = (L.u * sL + R.u * sR) / (sL + sR);
-	double vT = (L.v * sL + R.v * sR) / (sL + sR);
-	double wT = (L.w * sL + R.w * sR) / (sL + sR);
-	double hT = (L.H * sL + R.H * sR) / (sL + sR);
+	const double den = sL + sR;
+	const double yden = MK ? 1.0 / den : 0.0;
+	double uT = quot<MK>(L.u * sL + R.u * sR, den, yden);
+	double vT = quot<MK>(L.v * sL + R.v * sR, den, yden);
+	double wT = quot<MK>(L.w * sL + R.w * sR, den, yden);
+	double hT = quot<MK>(L.H * sL + R.H * sR, den, yden);
 	double q2 = uT * uT + vT * vT + wT * wT;
 	double cT = sqrt((gamma - 1.0) * (hT - q2 * 0.5));
 	double VT = uT * nx + vT * ny + wT * nz;
@@ -132,10 +154,11 @@ __device__ __forceinline__ void roeFluxEmit(const Prim &L, const Prim &R, double
 	double kr = (R.rho - L.rho) - dP / (cT * cT);
 	double aV = fabs(VT);
 	double rVR = R.rho * VR, lVL = L.rho * VL;
+	const double yc = MK ? 1.0 / c2x2 : 0.0;
 	auto comp = [&](int k, double avg, double e1, double e2, double t2, double e5) {
-		double f1 = e1 * l1 * k1 / c2x2;
+		double f1 = quot<MK>(e1 * l1 * k1, c2x2, yc);
 		double f234 = (e2 * kr + t2) * aV;
-		double f5 = e5 * l5 * k5 / c2x2;
+		double f5 = quot<MK>(e5 * l5 * k5, c2x2, yc);
 		emit(k, avg - 0.5 * (f1 + f234 + f5));
 	};
 	comp(0, (rVR + lVL) * 0.5, 1.0, 1.0, 0.0, 1.0);
@@ -429,7 +452,7 @@ __device__ __forceinline__ int logicalBlock(int w, int nBlocks, int xcds) {
 // and its halo (the outside cells its faces read) are staged in LDS as rho, u, v, w, p, H; the
 // update's inputs (W0, V, sum|S|, face slots) are loaded into registers. The flux loop then reads
 // LDS only, except for boundary faces, whose ghost cells live in global memory.
-template<int SCHEME, bool NT>
+template<int SCHEME, bool NT, bool MK>
 __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out, DevFree q,
                                                       double *W0, double alpha, int firstStage, int machDerived,
                                                       double *rmsPartial, int xcds) {
@@ -515,7 +538,7 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 		const Prim L = lprim(l0);
 		auto emit = [&](int k, double f) { lf[k * ms + j] = f * S; };
 		if (SCHEME == 0 && kind == 0) {
-			roeFluxEmit(L, lprim(l1), nx, ny, nz, q.gamma, emit);
+			roeFluxEmit<MK>(L, lprim(l1), nx, ny, nz, q.gamma, emit);
 			continue;
 		}
 		double F[5];
@@ -590,15 +613,18 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 		double ly = 0.5 * (fabs(v) + c) * sy;
 		double lz = 0.5 * (fabs(w) + c) * sz;
 		double dt = q.cfl * vol / (lx + ly + lz);
-		double nr = w0[0] - alpha * dt * r[0] / vol;
-		double nru = w0[1] - alpha * dt * r[1] / vol;
-		double nrv = w0[2] - alpha * dt * r[2] / vol;
-		double nrw = w0[3] - alpha * dt * r[3] / vol;
-		double nrE = w0[4] - alpha * dt * r[4] / vol;
-		double nu = nru / nr, nv = nrv / nr, nw = nrw / nr, nE = nrE / nr;
+		const double yv = MK ? 1.0 / vol : 0.0;
+		double nr = w0[0] - quot<MK>(alpha * dt * r[0], vol, yv);
+		double nru = w0[1] - quot<MK>(alpha * dt * r[1], vol, yv);
+		double nrv = w0[2] - quot<MK>(alpha * dt * r[2], vol, yv);
+		double nrw = w0[3] - quot<MK>(alpha * dt * r[3], vol, yv);
+		double nrE = w0[4] - quot<MK>(alpha * dt * r[4], vol, yv);
+		const double yr = MK ? 1.0 / nr : 0.0;
+		double nu = quot<MK>(nru, nr, yr), nv = quot<MK>(nrv, nr, yr), nw = quot<MK>(nrw, nr, yr);
+		double nE = quot<MK>(nrE, nr, yr);
 		double g = q.gamma;
 		double np = (g - 1) * nr * (nE - (nu * nu + nv * nv + nw * nw) * 0.5);
-		double nH = nE + (np / nr);
+		double nH = nE + quot<MK>(np, nr, yr);
 		double2 *oc = reinterpret_cast<double2 *>(cellPtr(out, e));
 		oc[0] = make_double2(nr, nu);
 		oc[1] = make_double2(nv, nw);
@@ -705,16 +731,20 @@ void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState 
 	size_t words = (size_t) 6 * (kBlock + p.maxHalo) + (size_t) 5 * p.maxSlots;
 	size_t lds = sizeof(double) * (words > (size_t) kBlock ? words : (size_t) kBlock);
 	const int x = t.xcds;
-#define E3D_LAUNCH(S, N)                                                                                          \
-	hipLaunchKernelGGL((kStageFused<S, N>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, q, W0, alpha, \
+#define E3D_LAUNCH(S, N, K)                                                                                           \
+	hipLaunchKernelGGL((kStageFused<S, N, K>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, q, W0, alpha, \
 	                   firstStage, machDerived, rmsPartial, x)
+#define E3D_LAUNCH_K(S, N)               \
+	if (t.sharedRecip) E3D_LAUNCH(S, N, true); \
+	else E3D_LAUNCH(S, N, false)
 	if (scheme == 1) {
-		if (t.nontemporal) E3D_LAUNCH(1, true);
-		else E3D_LAUNCH(1, false);
+		if (t.nontemporal) E3D_LAUNCH_K(1, true);
+		else E3D_LAUNCH_K(1, false);
 	} else {
-		if (t.nontemporal) E3D_LAUNCH(0, true);
-		else E3D_LAUNCH(0, false);
+		if (t.nontemporal) E3D_LAUNCH_K(0, true);
+		else E3D_LAUNCH_K(0, false);
 	}
+#undef E3D_LAUNCH_K
 #undef E3D_LAUNCH
 }
 
