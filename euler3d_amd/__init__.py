@@ -4,4 +4,4 @@ The product is libe3d_hip.so (HIP kernels for gfx950 behind the C ABI of include
 E3D_Solver executable built on it; this package binds that library for Python callers.
 """
 from .native import E3DError, Params, default_params, version  # noqa: F401
-from .solver import EulerSolver, Mesh, SimConfig, STATE_FIELDS  # noqa: F401
+from .solver import Batch, EulerSolver, Mesh, SimConfig, STATE_FIELDS  # noqa: F401

@@ -28,8 +28,8 @@ OBJ = ROOT / "_build" / "obj"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 ARCH = os.environ.get("E3D_OFFLOAD_ARCH", "gfx950")
 
-HOST_SRCS = ["mesh_io.cpp", "config.cpp", "preprocess.cpp", "generators.cpp", "plan.cpp", "capi.cpp"]
-DEVICE_SRCS = ["kernels.hip"]
+HOST_SRCS = ["mesh_io.cpp", "config.cpp", "preprocess.cpp", "generators.cpp", "plan.cpp", "capi.cpp", "batch.cpp"]
+DEVICE_SRCS = ["kernels.hip", "kernels_mixed.hip"]
 
 CXXFLAGS = ["-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
             "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{ROOT}/include"]

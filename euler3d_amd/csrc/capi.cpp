@@ -1,6 +1,7 @@
 // C ABI of libe3d_hip.so: mesh and configuration handling on the host, device context that keeps
 // one case resident in HBM and advances it with the kernels of kernels.hip.
 #include "../../include/e3d.h"
+#include "abi_util.hpp"
 #include "device.hpp"
 #include "e3d_host.hpp"
 #include "plan.hpp"
@@ -16,35 +17,18 @@
 #include <string>
 #include <vector>
 
+std::string &e3d_abi::lastError() {
+	thread_local std::string msg;
+	return msg;
+}
+
 namespace {
 
-thread_local std::string g_lastError;
-
-int fail(int code, const std::string &msg) {
-	g_lastError = msg;
-	return code;
-}
-
-struct HipError : std::runtime_error {
-	using std::runtime_error::runtime_error;
-};
-
-void hipCheck(hipError_t e, const char *what) {
-	if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template<typename F>
-int guarded(F &&f) {
-	try {
-		return f();
-	} catch (const HipError &e) {
-		return fail(E3D_ERR_DEVICE, e.what());
-	} catch (const std::invalid_argument &e) {
-		return fail(E3D_ERR_ARG, e.what());
-	} catch (const std::exception &e) {
-		return fail(E3D_ERR_MESH, e.what());
-	}
-}
+using e3d_abi::fail;
+using e3d_abi::guarded;
+using e3d_abi::hipCheck;
+using e3d_abi::HipError;
+using e3d_abi::fillFree;
 
 template<typename T>
 struct DBuf {
@@ -67,12 +51,6 @@ struct DBuf {
 };
 
 }// namespace
-
-struct e3d_mesh {
-	e3d::MeshFile file;
-	e3d::SolverMesh sm;
-	bool preprocessed = false;
-};
 
 struct e3d_ctx {
 	int device = 0;
@@ -120,22 +98,6 @@ struct e3d_ctx {
 };
 
 namespace {
-
-void fillFree(const e3d_params &p, e3d::DevFree &q) {
-	// FlowField::Initialize, FlowField.cpp:72-81 (rho_inf = p_inf = 1)
-	const double pi = 3.141592653589793238;
-	double g = p.gamma;
-	q.gamma = g;
-	q.cfl = p.cfl;
-	q.rho_inf = 1.0;
-	q.p_inf = 1.0;
-	q.M_inf = p.mach;
-	q.u_inf = q.M_inf * std::sqrt(g) * std::cos(p.aoa_deg * (pi / 180));
-	q.v_inf = q.M_inf * std::sqrt(g) * std::sin(p.aoa_deg * (pi / 180));
-	q.w_inf = 0.0;
-	q.E_inf = q.p_inf / ((g - 1) * q.rho_inf) + ((q.u_inf * q.u_inf + q.v_inf * q.v_inf + q.w_inf * q.w_inf) / 2.0);
-	q.H_inf = q.E_inf + (q.p_inf / q.rho_inf);
-}
 
 void initFreestream(e3d_ctx *c) {
 	std::vector<double> h(c->nTot);
@@ -207,7 +169,7 @@ void runIteration(e3d_ctx *c, double *dsum) {
 
 extern "C" {
 
-const char *e3d_last_error(void) { return g_lastError.c_str(); }
+const char *e3d_last_error(void) { return e3d_abi::lastError().c_str(); }
 const char *e3d_version(void) { return "e3d-mi355x 0.1 (gfx950, FP64 plain path)"; }
 
 int e3d_params_default(e3d_params *out) {

@@ -1,0 +1,103 @@
+// Mixed-precision multi-case solver: device-side case table, block table and the launchers of the
+// kernels that advance every resident case together.
+//
+// Precision split. Face fluxes, boundary ghosts and the local time step run in FP32 on FP32
+// copies of the state and of the face geometry; the conservative state W, the RK update, the
+// per-cell residual accumulation (Kahan-compensated), the residual norms and the force
+// integration stay FP64.
+//
+// Perturbation form. The FP32 copy of a cell holds its primitives minus the freestream
+// (d_rho, d_u, d_v, d_w, d_p, d_H) and each face contributes F - F_inf(n) instead of F, with
+// F_inf(n) the freestream flux through the face. The subtracted terms sum to F_inf . sum(n S) = 0
+// over a closed cell, so the residual is unchanged in exact arithmetic, while the FP32 rounding
+// error scales with the local departure from freestream instead of with |F|. That keeps the
+// residual floor of the FP32 flux far below the 1e-6 drop the cases converge to.
+//
+// Cases. Every resident case owns its arrays (plan, metrics, W, two FP32 record buffers,
+// per-block partial sums, residual history). A device table of MixCase descriptors and a table
+// of (case, block) pairs let one persistent launch walk the blocks of all cases; a case's
+// arithmetic does not depend on which other cases are resident or where its blocks sit in the
+// table, so its result is the same alone and in any mix.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace e3d {
+
+constexpr int kMixBlock = 256;  // cells per block = threads per workgroup
+constexpr int kRecWidth = 8;    // floats per FP32 cell record
+
+// FP32 cell record (32 B): perturbations from freestream and the stored Mach number
+enum RecField : int { R_DRHO = 0, R_DU, R_DV, R_DW, R_DP, R_DH, R_M, R_PAD };
+
+struct MixCase {
+	// sizes
+	int nElem, nGhost, nBlocks, csrStride;
+	int scheme; // 0 Roe, 1 AUSM
+	int histCap;
+	int maxIter;
+	int pad0;
+	double drop;        // converged when rms < drop * rms of the first iteration
+	double minResidual; // or when rms < minResidual (absolute)
+	// block plan
+	const int *blockSlot, *blockHalo, *haloCell, *ghostOf;
+	const uint32_t *pack;
+	const float *nx, *ny, *nz, *S;
+	const uint16_t *csr;
+	// cell metrics (FP32) and volume
+	const float *vol, *sx, *sy, *sz;
+	// state
+	double *W;      // 5 x nElem conservative state (rho, rho u, rho v, rho w, rho E), FP64
+	float *rec[2];  // (nElem + nGhost) x kRecWidth FP32 records, ping-pong
+	double *partial; // nBlocks partial sums of R_rho^2 (last stage)
+	double *hist;    // residual RMS per iteration, histCap entries
+	// freestream (FP32 for the flux, FP64 for the update)
+	float gamma, cfl, rInf, uInf, vInf, wInf, pInf, hInf, mInf;
+	float pad1;
+	double gammaD, rInfD, uInfD, vInfD, wInfD, pInfD, hInfD;
+};
+
+// device-written progress of a case
+struct MixStatus {
+	int parity; // FP32 record buffer holding the state at the start of an iteration
+	int iter;   // iterations done
+	int done;   // 1: converged or out of iterations (its blocks are skipped)
+	int active; // 0: the slot holds no case
+	double rms0, rmsLast;
+};
+
+// launch choices of the mixed stage (they change speed only, never results)
+struct MixTuning {
+	int wgPerCU = 4;        // persistent workgroups per compute unit
+	int xcds = 8;           // XCD count for the L2-aware block order
+	bool nontemporal = true;
+};
+
+struct MixLaunch {
+	const MixCase *cases;   // case table (device)
+	MixStatus *status;      // per case slot (device)
+	const int2 *blocks;     // (case slot, block of the case)
+	const int *nBlocks;     // number of entries of `blocks` in use (device scalar)
+	int nSlots;             // case slots
+	int grid;               // persistent workgroups (a multiple of the XCD count)
+	int cap;                // LDS cells per block: kMixBlock + the largest halo of any resident case
+	int ms;                 // LDS face slots per block: the largest slot count, even
+	int schemes;            // bit s set: some resident case uses flux scheme s (one launch each)
+};
+
+// LDS bytes of a block plan with these extremes
+size_t mixLdsBytes(int maxSlots, int maxHalo);
+// one RK stage (0..4) of every active, unfinished case
+void launchMixStage(const MixLaunch &L, int stage, const MixTuning &t, hipStream_t st);
+// after the 5 stages: per-case RMS, history, convergence, buffer parity
+void launchMixIterEnd(const MixLaunch &L, hipStream_t st);
+// FP64 primitives (rho, u, v, w, p, E, H, M) of the interior cells from W: out[8][nElem]
+void launchMixPrimitives(const MixCase &hc, double *out, hipStream_t st);
+// wall-face force/moment of one case from its FP64 state (same reduction as the FP64 path)
+void launchMixAero(int nWall, const int *wallCell, const double *Nx, const double *Ny, const double *Nz,
+                   const double *Cx, const double *Cy, const double *Cz, const MixCase *dc, double cpFactor,
+                   double pInf, double cref, double sref, double *out6, hipStream_t st);
+
+}// namespace e3d

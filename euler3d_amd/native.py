@@ -14,7 +14,8 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libe3d_hip.so"
 
 E3D_OK = 0
-ERRORS = {1: "argument", 2: "io", 3: "mesh", 4: "device", 5: "unsupported"}
+ERRORS = {1: "argument", 2: "io", 3: "mesh", 4: "device", 5: "unsupported", 6: "busy"}
+E3D_ERR_BUSY = 6
 
 
 class E3DError(RuntimeError):
@@ -37,6 +38,15 @@ class Params(C.Structure):
 class MeshInfo(C.Structure):
     _fields_ = [(n, C.c_int) for n in
                 ("n_node", "n_elem", "n_face", "n_face_int", "n_ghost", "n_wall", "n_far", "n_sym")]
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_double), ("max_cases", C.c_int), ("wg_per_cu", C.c_int), ("xcds", C.c_int),
+                ("nontemporal", C.c_int), ("use_graph", C.c_int)]
+
+
+class CaseOpts(C.Structure):
+    _fields_ = [("drop", C.c_double), ("max_iter", C.c_int), ("hist_cap", C.c_int)]
 
 
 _lib = None
@@ -76,6 +86,20 @@ _SIGS = {
     "e3d_set_kernel_timing": (C.c_int, [P, C.c_int]),
     "e3d_kernel_timing": (C.c_int, [P, DP, IP, C.c_char_p, C.c_int]),
     "e3d_destroy": (None, [P]),
+    "e3d_batch_opts_default": (C.c_int, [C.POINTER(BatchOpts)]),
+    "e3d_batch_create": (C.c_int, [C.c_int, C.POINTER(BatchOpts), PP]),
+    "e3d_batch_case_bytes": (C.c_int, [P, C.POINTER(CaseOpts), DP]),
+    "e3d_batch_add": (C.c_int, [P, P, C.POINTER(Params), C.POINTER(CaseOpts), IP]),
+    "e3d_batch_run": (C.c_int, [P, C.c_int]),
+    "e3d_batch_wait": (C.c_int, [P]),
+    "e3d_batch_status": (C.c_int, [P, C.c_int, IP, IP, DP, DP]),
+    "e3d_batch_result": (C.c_int, [P, C.c_int, DP, C.c_int, IP, DP, DP, DP]),
+    "e3d_batch_get_state": (C.c_int, [P, C.c_int, C.POINTER(DP)]),
+    "e3d_batch_remove": (C.c_int, [P, C.c_int]),
+    "e3d_batch_info": (C.c_int, [P, IP, DP, DP]),
+    "e3d_batch_set_timing": (C.c_int, [P, C.c_int]),
+    "e3d_batch_timing": (C.c_int, [P, DP, IP, IP]),
+    "e3d_batch_destroy": (None, [P]),
 }
 
 EXPORTED = tuple(_SIGS)

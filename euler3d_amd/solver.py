@@ -233,3 +233,113 @@ class EulerSolver:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Mixed-precision multi-case engine: many cases resident on one device, advanced together by
+    one persistent launch per RK stage (no counterpart in the reference, which solves one case per
+    process). A case's result does not depend on the other resident cases."""
+
+    def __init__(self, device: int = 0, hbm_budget: float = 0.0, max_cases: int = 256, wg_per_cu: int = 4,
+                 xcds: int = 8, nontemporal: bool = True, use_graph: bool = True):
+        o = N.BatchOpts()
+        N.check(N.lib().e3d_batch_opts_default(C.byref(o)))
+        o.hbm_budget = float(hbm_budget)
+        o.max_cases = int(max_cases)
+        o.wg_per_cu = int(wg_per_cu)
+        o.xcds = int(xcds)
+        o.nontemporal = int(nontemporal)
+        o.use_graph = int(use_graph)
+        h = C.c_void_p()
+        N.check(N.lib().e3d_batch_create(int(device), C.byref(o), C.byref(h)))
+        self._h = h
+        self._n = {}
+
+    @staticmethod
+    def case_opts(drop: float = 1e-6, max_iter: int = 5000, hist_cap: int | None = None) -> N.CaseOpts:
+        co = N.CaseOpts()
+        co.drop = float(drop)
+        co.max_iter = int(max_iter)
+        co.hist_cap = int(max_iter if hist_cap is None else hist_cap)
+        return co
+
+    @staticmethod
+    def case_bytes(mesh: Mesh, drop: float = 1e-6, max_iter: int = 5000) -> float:
+        b = C.c_double()
+        co = Batch.case_opts(drop, max_iter)
+        N.check(N.lib().e3d_batch_case_bytes(mesh._h, C.byref(co), C.byref(b)))
+        return b.value
+
+    def add(self, mesh: Mesh, params: N.Params, drop: float = 1e-6, max_iter: int = 5000,
+            hist_cap: int | None = None) -> int:
+        """Admit a case; returns its id. Raises E3DError with code E3D_ERR_BUSY when full."""
+        co = self.case_opts(drop, max_iter, hist_cap)
+        cid = C.c_int()
+        N.check(N.lib().e3d_batch_add(self._h, mesh._h, C.byref(params), C.byref(co), C.byref(cid)))
+        info = mesh.info()
+        self._n[cid.value] = (info["n_elem"], info["n_ghost"], co.hist_cap)
+        return cid.value
+
+    def run(self, n: int) -> None:
+        N.check(N.lib().e3d_batch_run(self._h, int(n)))
+
+    def wait(self) -> None:
+        N.check(N.lib().e3d_batch_wait(self._h))
+
+    def status(self, cid: int) -> dict:
+        done, it = C.c_int(), C.c_int()
+        r0, r1 = C.c_double(), C.c_double()
+        N.check(N.lib().e3d_batch_status(self._h, int(cid), C.byref(done), C.byref(it), C.byref(r0), C.byref(r1)))
+        return {"done": bool(done.value), "iterations": it.value, "rms0": r0.value, "rms": r1.value}
+
+    def result(self, cid: int) -> dict:
+        cap = self._n[cid][2]
+        hist = np.empty(max(cap, 1))
+        nh = C.c_int()
+        coeffs, cf, cm = np.empty(3), np.empty(3), np.empty(3)
+        N.check(N.lib().e3d_batch_result(self._h, int(cid), N.dptr(hist), cap, C.byref(nh), N.dptr(coeffs),
+                                         N.dptr(cf), N.dptr(cm)))
+        return {"rms": hist[: nh.value].copy(), "CL": coeffs[0], "CD": coeffs[1], "CM": coeffs[2],
+                "force_coeff": cf, "moment_coeff": cm}
+
+    def get_state(self, cid: int) -> dict:
+        n_elem, n_ghost, _ = self._n[cid]
+        out = {k: np.empty(n_elem + n_ghost) for k in STATE_FIELDS}
+        ptrs = (N.DP * 8)(*[N.dptr(out[k]) for k in STATE_FIELDS])
+        N.check(N.lib().e3d_batch_get_state(self._h, int(cid), ptrs))
+        return out
+
+    def remove(self, cid: int) -> None:
+        N.check(N.lib().e3d_batch_remove(self._h, int(cid)))
+        self._n.pop(cid, None)
+
+    def info(self) -> dict:
+        n, used, budget = C.c_int(), C.c_double(), C.c_double()
+        N.check(N.lib().e3d_batch_info(self._h, C.byref(n), C.byref(used), C.byref(budget)))
+        return {"cases": n.value, "bytes": used.value, "budget": budget.value}
+
+    def set_timing(self, enable: bool) -> None:
+        N.check(N.lib().e3d_batch_set_timing(self._h, int(enable)))
+
+    def timing(self) -> tuple[float, int, bool]:
+        """(mean ms per timed launch, launches, whether a launch is a whole iteration)"""
+        ms, n, per = C.c_double(), C.c_int(), C.c_int()
+        N.check(N.lib().e3d_batch_timing(self._h, C.byref(ms), C.byref(n), C.byref(per)))
+        return ms.value, n.value, bool(per.value)
+
+    def close(self) -> None:
+        if self._h:
+            N.lib().e3d_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

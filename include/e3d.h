@@ -42,6 +42,7 @@ extern "C" {
 #define E3D_ERR_MESH 3
 #define E3D_ERR_DEVICE 4
 #define E3D_ERR_UNSUPPORTED 5
+#define E3D_ERR_BUSY 6 /* batch: no free case slot or the HBM budget is exhausted; retry after a removal */
 
 #define E3D_SCHEME_ROE 0
 #define E3D_SCHEME_AUSM 1
@@ -143,6 +144,68 @@ int e3d_last_elapsed_ms(e3d_ctx *c, double *ms);
 int e3d_set_kernel_timing(e3d_ctx *c, int enable);
 int e3d_kernel_timing(e3d_ctx *c, double *avg_ms, int *launches, char *name, int name_len);
 void e3d_destroy(e3d_ctx *c);
+
+/* ---- mixed-precision multi-case engine (batched and served solves) ------------------------
+ * No counterpart in the reference, which runs one case per process (src/Solver.cpp:19-73). Many
+ * cases of any sizes stay resident on one device and advance together: one persistent kernel
+ * launch per RK stage walks the blocks of every resident case, and one iteration (5 stages plus
+ * the per-case RMS and convergence test) is replayed from a captured HIP graph. Face fluxes,
+ * ghosts and local time steps are FP32 (perturbation form, on FP32 copies of the state and the
+ * face geometry); the conservative state, the RK update, the Kahan-compensated residual
+ * accumulation, the residual norms and the force integration are FP64. A case's arithmetic does
+ * not depend on the other resident cases, so its result is bitwise the same alone and in a mix.
+ * Cases are admitted while others run, against the HBM budget and the slot count, and retired
+ * without draining the batch. A converged case stops advancing on the device by itself. */
+typedef struct e3d_batch e3d_batch;
+
+typedef struct e3d_batch_opts {
+	double hbm_budget;   /* bytes the resident cases may hold; <= 0: 90 % of the free device memory */
+	int max_cases;       /* case slots */
+	int wg_per_cu;       /* persistent workgroups per compute unit (launch shape; results unchanged) */
+	int xcds;            /* XCD count of the L2-aware block order, 1 = dispatch order */
+	int nontemporal;     /* non-temporal loads for read-once arrays */
+	int use_graph;       /* replay iterations from a captured HIP graph */
+} e3d_batch_opts;
+
+typedef struct e3d_case_opts {
+	double drop;         /* converged when rms < drop * rms of the first iteration (0: off) */
+	int max_iter;        /* iteration cap */
+	int hist_cap;        /* residual RMS history entries kept on the device */
+} e3d_case_opts;
+
+int e3d_batch_opts_default(e3d_batch_opts *out);
+int e3d_batch_create(int device, const e3d_batch_opts *opts, e3d_batch **out);
+/* device bytes a case of this preprocessed mesh holds while resident */
+int e3d_batch_case_bytes(const e3d_mesh *m, const e3d_case_opts *co, double *bytes);
+/* Admits a case (freestream start, FlowField::Initialize) and returns its id. The mesh may be
+ * freed afterwards. E3D_ERR_BUSY when no slot is free or the case would exceed the budget. The
+ * upload overlaps the iterations already queued; the case starts with the next e3d_batch_run. */
+int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3d_case_opts *co, int *case_id);
+/* Queues n iterations of every resident, unfinished case and returns without waiting. */
+int e3d_batch_run(e3d_batch *b, int n);
+/* Waits for the queued iterations. */
+int e3d_batch_wait(e3d_batch *b);
+/* Waits, then reports a case's progress: done (converged or out of iterations), iterations,
+ * RMS of the first and of the last iteration. */
+int e3d_batch_status(e3d_batch *b, int case_id, int *done, int *iters, double *rms0, double *rms_last);
+/* Residual RMS history (one value per iteration, at most cap) and the force and moment
+ * coefficients of the current state: coeffs = CL, CD, CM (AeroCoefficients::Update,
+ * src/solver/AeroCoefficients.cpp:38-52, FP64). */
+int e3d_batch_result(e3d_batch *b, int case_id, double *hist, int cap, int *n_hist, double coeffs[3], double cf[3],
+                     double cm[3]);
+/* FP64 state of a case in the FlowField numbering as e3d_get_state gives it: interior cells from
+ * the FP64 conservative state, ghost cells from their FP32 copies */
+int e3d_batch_get_state(e3d_batch *b, int case_id, double *const fields[8]);
+/* Retires a case and releases its device memory (stream-ordered; no device-wide wait). */
+int e3d_batch_remove(e3d_batch *b, int case_id);
+/* resident cases, bytes they hold, budget */
+int e3d_batch_info(e3d_batch *b, int *n_cases, double *bytes, double *budget);
+/* Per-launch timing of the stage kernel over the next e3d_batch_run calls (HIP events around
+ * each stage when graphs are off, around each iteration otherwise); same contract as
+ * e3d_kernel_timing. */
+int e3d_batch_set_timing(e3d_batch *b, int enable);
+int e3d_batch_timing(e3d_batch *b, double *avg_ms, int *launches, int *per_iteration);
+void e3d_batch_destroy(e3d_batch *b);
 
 #ifdef __cplusplus
 }

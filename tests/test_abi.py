@@ -44,3 +44,25 @@ def test_solver_executable_usage():
     exe = ROOT / "euler3d_amd" / "bin" / "E3D_Solver"
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     assert r.returncode != 0 and "Usage" in r.stderr
+
+
+def test_batch_case_bytes_and_argument_checks():
+    """Footprint bound and option validation of the batch engine (host only, no device call)."""
+    import pytest
+
+    from euler3d_amd import Batch, Mesh
+
+    m = Mesh.naca(ni=33, nj=17, nk=2).preprocess(1.0)
+    b1 = Batch.case_bytes(m, max_iter=100)
+    b2 = Batch.case_bytes(m, max_iter=10000)
+    assert b1 > 0 and b2 - b1 >= 8 * 9900 - 256  # the residual history is part of the footprint
+    lib = native.lib()
+    o = native.BatchOpts()
+    assert lib.e3d_batch_opts_default(ctypes.byref(o)) == 0
+    assert o.max_cases > 0 and o.wg_per_cu > 0 and o.use_graph == 1
+    o.max_cases = 0
+    h = ctypes.c_void_p()
+    assert lib.e3d_batch_create(0, ctypes.byref(o), ctypes.byref(h)) == 1
+    assert "max_cases" in lib.e3d_last_error().decode()
+    with pytest.raises(native.E3DError):
+        native.check(lib.e3d_batch_case_bytes(Mesh.box(2, 2, 2)._h, None, ctypes.byref(ctypes.c_double())))
