@@ -51,6 +51,44 @@ def measured_traffic(kernel: str, n_cells: int, scheme: int):
     return rec["fetch_kib"] * rec["fetch_scale"] * 1024 + rec["write_kib"] * rec["write_scale"] * 1024
 
 
+def mixed_bytes_per_cell_stage(n_cells: int, n_faces: int) -> float:
+    # FP32 state copy in (20) + W0 FP64 in (40) + W FP64 out (40) + FP32 copy out (20) + FP32 cell
+    # metrics (16) + per face: FP32 area vector + two int32 ids (20)
+    return 136.0 + 20.0 * n_faces / n_cells
+
+
+def mixed_precision_leg(mesh, prm, n_cells: int, n_faces: int, steps: int, warmup: int) -> dict:
+    """The same workload through the mixed-precision engine (FP32 flux/BC, FP64 update): wall-clock
+    throughput with graph replay, stage-kernel mean from HIP events in a second, ungraphed pass."""
+    from euler3d_amd import Batch
+
+    out = {}
+    with Batch() as b:
+        cid = b.add(mesh, prm, drop=0.0, max_iter=1 << 30, hist_cap=4 * (steps + warmup))
+        b.run(warmup)
+        b.wait()
+        t0 = time.perf_counter()
+        b.run(steps)
+        b.wait()
+        wall = time.perf_counter() - t0
+        out["value"] = n_cells * 5 * steps / wall
+        out["ms_per_step"] = wall * 1e3 / steps
+        it = b.status(cid)["iterations"]
+    with Batch(use_graph=False) as b:
+        b.add(mesh, prm, drop=0.0, max_iter=1 << 30, hist_cap=4 * (steps + warmup))
+        b.run(warmup)
+        b.set_timing(True)
+        b.run(steps)
+        kern_ms, launches, _ = b.timing()
+    bpc = mixed_bytes_per_cell_stage(n_cells, n_faces)
+    achieved = bpc * n_cells / (kern_ms * 1e-3) / 1e9
+    out.update({"unit": "cell-stage/s", "dtype": "f32 flux / f64 update", "iterations": it,
+                "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": achieved / HBM_PEAK_GBS, "kernel": "kMixStage", "kernel_ms": kern_ms,
+                             "bytes_per_cell_stage": bpc, "launches": launches}})
+    return out
+
+
 def cpu_baseline(mesh, params: dict, n_cells: int) -> dict:
     sys.path.insert(0, str(ROOT / "oracle"))
     from oracle_ctypes import Oracle
@@ -67,6 +105,115 @@ def cpu_baseline(mesh, params: dict, n_cells: int) -> dict:
                       f"restatement compiled -O2, single thread"}
 
 
+def build_meshes(cases, workers: int = 8):
+    """Generate and preprocess the meshes of `cases` in parallel (the native calls release the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(lambda c: c.build(), cases))
+
+
+def bench_batch(a) -> dict:
+    """64 uneven airfoil cases: all resident in the mixed-precision engine (persistent launch,
+    graph-replayed iterations, autotuned launch shape) versus the plain FP64 path one case at a
+    time. Each case runs to a 1e-6 residual drop or the iteration cap."""
+    from euler3d_amd import Batch
+    from euler3d_amd.autotune import autotune
+    from euler3d_amd.serve import plain_solve
+    from euler3d_amd.workloads import airfoil_batch
+
+    cases = airfoil_batch(a.cases, seed=0)
+    t = time.perf_counter()
+    built = build_meshes(cases)
+    cells = [m.info()["n_elem"] for m, _ in built]
+    log(f"batch: {len(cases)} cases, {sum(cells)} cells, meshes in {time.perf_counter() - t:.1f} s")
+    tune = autotune(built, iters=3, warmup=1, use_cache=False)
+    log(f"autotune: best {tune['best']} ({len(tune['table'])} shapes)")
+    with Batch(max_cases=len(cases), wg_per_cu=tune["best"][0], xcds=tune["best"][1],
+               nontemporal=tune["best"][2]) as b:
+        t0 = time.perf_counter()
+        ids = [b.add(m, p, drop=a.drop, max_iter=a.max_iter) for m, p in built]
+        t_admit = time.perf_counter() - t0
+        while True:
+            b.run(100)
+            st = b.poll()
+            if all(s["done"] for s in st):
+                break
+        wall = time.perf_counter() - t0
+        res = [b.result(i) for i in ids]
+        iters = [len(r["rms"]) for r in res]
+    cell_stages = 5 * sum(c * i for c, i in zip(cells, iters))
+    log(f"batched mixed: {wall:.2f} s ({t_admit:.2f} s admission), {cell_stages / wall:.3e} cell-stage/s")
+    t0 = time.perf_counter()
+    plain = [plain_solve(m, p, drop=a.drop, max_iter=a.max_iter) for m, p in built]
+    wall_plain = time.perf_counter() - t0
+    log(f"plain fp64 one at a time: {wall_plain:.2f} s")
+    dcl = max(abs(r["CL"] - q["CL"]) for r, q in zip(res, plain))
+    dcd = max(abs(r["CD"] - q["CD"]) for r, q in zip(res, plain))
+    return {
+        "metric": "batched solves/h", "value": 3600.0 * len(cases) / wall, "unit": "solves/h", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f32 flux / f64 update", "data": "synthetic",
+        "config": {"workload": f"{len(cases)} NACA 4-digit O-grid cases (n~U{{65..257}} nodes a side, t U[0.06,0.15], "
+                               f"camber U[0,0.04], M U[0.3,0.85], AoA U[-2,4] deg, seed 0), Roe, RK5 CFL 1.5, "
+                               f"to a {a.drop:g} residual drop or {a.max_iter} iterations",
+                   "cells_total": sum(cells), "launch_shape": list(tune["best"])},
+        "wall_s": wall, "admission_s": t_admit, "cell_stage_per_s": cell_stages / wall,
+        "iterations": {"min": min(iters), "max": max(iters), "mean": sum(iters) / len(iters)},
+        "plain_fp64": {"wall_s": wall_plain, "solves_per_h": 3600.0 * len(cases) / wall_plain,
+                       "iterations_mean": sum(q["iterations"] for q in plain) / len(plain)},
+        "speedup_vs_plain": wall_plain / wall,
+        "max_abs_dCL_vs_fp64": dcl, "max_abs_dCD_vs_fp64": dcd,
+        "autotune": tune["table"],
+    }
+
+
+def bench_serve(a) -> list[dict]:
+    """Poisson request streams over a fixed seeded request set (three mesh families, 10k-2M
+    cells): the continuous-batching server on the mixed-precision engine against the plain FP64
+    path serving one request at a time in arrival order (its latencies follow from its measured
+    service times). Meshes are generated and preprocessed before the clock starts."""
+    from euler3d_amd.serve import Server, fifo_latencies, percentile, plain_solve, summarize
+    from euler3d_amd.workloads import poisson_arrivals, request_mix
+
+    cases = request_mix(a.requests, seed=0, max_cells=a.max_cells)
+    t = time.perf_counter()
+    built = build_meshes(cases)
+    cells = [m.info()["n_elem"] for m, _ in built]
+    log(f"serve: {len(cases)} requests, {sum(cells)} cells ({min(cells)}..{max(cells)}), meshes in "
+        f"{time.perf_counter() - t:.1f} s")
+    plain = []
+    for i, (m, p) in enumerate(built):
+        plain.append(plain_solve(m, p, drop=a.drop, max_iter=a.max_iter))
+        log(f"plain {i}: {cases[i].kind} {cells[i]} cells, {plain[i]['service']:.2f} s, {plain[i]['iterations']} it")
+    services = [q["service"] for q in plain]
+    lines = []
+    srv = Server(drop=a.drop, max_iter=a.max_iter, hbm_budget=200e9)
+    try:
+        for lam in a.lambdas:
+            arr = poisson_arrivals(len(cases), lam, seed=0)
+            rec = srv.serve([(m, p, cases[i].kind) for i, (m, p) in enumerate(built)], arr)
+            sv = summarize(rec)
+            lat, span = fifo_latencies(arr, services)
+            pl = {"p50_s": percentile(lat, 50), "p99_s": percentile(lat, 99), "makespan_s": span,
+                  "solves_per_h": 3600.0 * len(cases) / span}
+            dcl = max(abs(r["CL"] - q["CL"]) for r, q in zip(rec, plain))
+            dcd = max(abs(r["CD"] - q["CD"]) for r, q in zip(rec, plain))
+            line = {"metric": "served solves/h", "value": sv["solves_per_h"], "unit": "solves/h", "n_gpus": 1,
+                    "higher_is_better": True, "dtype": "f32 flux / f64 update", "data": "synthetic",
+                    "config": {"workload": f"{len(cases)} requests (naca/bump/wing, 10k-{a.max_cells:.0f} cells, "
+                                           f"seed 0), Poisson rate {lam} req/s, {a.drop:g} drop or {a.max_iter} it, "
+                                           f"200 GB HBM budget", "rate": lam, "cells_total": sum(cells)},
+                    "served": sv, "plain_fp64": pl, "vs_plain_solves_per_h": sv["solves_per_h"] / pl["solves_per_h"],
+                    "max_abs_dCL_vs_fp64": dcl, "max_abs_dCD_vs_fp64": dcd}
+            log(f"rate {lam}: served p50 {sv['p50_s']:.2f} s p99 {sv['p99_s']:.2f} s {sv['solves_per_h']:.0f}/h; "
+                f"plain p50 {pl['p50_s']:.2f} s p99 {pl['p99_s']:.2f} s {pl['solves_per_h']:.0f}/h")
+            print(json.dumps(line), flush=True)
+            lines.append(line)
+    finally:
+        srv.close()
+    return lines
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,14 +224,31 @@ def main() -> None:
     ap.add_argument("--nz", type=int, default=64)
     ap.add_argument("--scheme", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run one plain device-to-device copy of this many MiB (a known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--mode", choices=("plain", "batch", "serve"), default="plain",
+                    help="plain: the FP64 stage benchmark (default); batch: 64 uneven airfoil cases; "
+                         "serve: Poisson request streams")
+    ap.add_argument("--cases", type=int, default=64)
+    ap.add_argument("--requests", type=int, default=24)
+    ap.add_argument("--max-cells", type=float, default=2e6)
+    ap.add_argument("--lambdas", type=float, nargs="+", default=[1, 2, 5, 10, 20, 50])
+    ap.add_argument("--drop", type=float, default=1e-6)
+    ap.add_argument("--max-iter", type=int, default=5000)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs one resident mesh per GPU and measures one GPU only; "
               f"--gpus {a.gpus} is not supported (multi-GPU sharding is a design note, not built)", file=sys.stderr)
         sys.exit(2)
+
+    if a.mode == "batch":
+        print(json.dumps(bench_batch(a)), flush=True)
+        return
+    if a.mode == "serve":
+        bench_serve(a)
+        return
 
     from euler3d_amd import EulerSolver, Mesh, default_params
 
@@ -145,6 +309,10 @@ def main() -> None:
         "cpu_baseline": None,
     }
     solver.close()
+    if not a.no_mixed:
+        out["mixed_precision"] = mixed_precision_leg(mesh, prm, n_cells, n_faces, a.steps, a.warmup)
+        log(f"mixed precision: {out['mixed_precision']['value']:.3e} cell-stage/s, kMixStage "
+            f"{out['mixed_precision']['roofline']['kernel_ms'] * 1e3:.1f} us")
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, pd, n_cells)
         log(f"cpu baseline: {out['cpu_baseline']['value']:.3e} cell-stage/s")

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,7 +27,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t) 255; }
 
 // sizes of one case's device arrays, in the order they are carved from its allocation
 struct CaseLayout {
-	size_t nB, nH, nS, nE, nG, stride, hist, nW;
+	size_t nB, nH, nS, nE, nG, stride, hist, nW, nDefect = 0;
 	size_t off[24];
 	size_t total = 0;
 	void build() {
@@ -39,7 +40,8 @@ struct CaseLayout {
 		    (nE + nG) * e3d::kRecWidth * 4, (nE + nG) * e3d::kRecWidth * 4,           // rec[2]
 		    std::max<size_t>(nB, 1) * 8, std::max<size_t>(hist, 1) * 8,               // partial hist
 		    std::max<size_t>(nW, 1) * 4, std::max<size_t>(nW, 1) * 8 * 6,             // wallCell wN wC
-		    8 * 8};                                                                   // scalars
+		    8 * 8,                                                                    // scalars
+		    std::max<size_t>(nDefect, 1) * 12};                                       // defect
 		total = 0;
 		for (size_t i = 0; i < sizeof(sz) / sizeof(sz[0]); i++) {
 			off[i] = total;
@@ -49,8 +51,8 @@ struct CaseLayout {
 };
 
 struct CaseHost {
-	bool used = false;
-	bool pending = false; // admitted, not yet in the device tables
+	bool used = false;    // slot taken (the case may still be uploading)
+	bool ready = false;   // uploaded; part of the device tables from the next activation on
 	int nElem = 0, nGhost = 0, nBlocks = 0, maxSlots = 0, maxHalo = 0;
 	e3d_params prm{};
 	e3d::DevFree fr{};
@@ -69,7 +71,7 @@ struct CaseHost {
 
 struct e3d_batch {
 	int device = 0;
-	hipStream_t stream = nullptr, upload = nullptr;
+	hipStream_t stream = nullptr;
 	e3d::MixTuning tune{};
 	bool useGraph = true;
 	int nSlots = 0, nCU = 0;
@@ -89,6 +91,8 @@ struct e3d_batch {
 	std::vector<CaseHost> cs;
 	std::vector<int> statusInit; // slots whose status must be (re)written at the next activation
 	bool dirty = false;
+	// guards the host-side tables: admissions may run on other threads while iterations run
+	std::mutex mu;
 	// launch description and the captured iteration
 	e3d::MixLaunch L{};
 	hipGraph_t graph = nullptr;
@@ -137,11 +141,10 @@ bool sameLaunch(const e3d::MixLaunch &a, const e3d::MixLaunch &c) {
 void activate(e3d_batch *b) {
 	if (!b->dirty) return;
 	b->sync(); // the pinned staging below may still be read by earlier copies
-	hipCheck(hipStreamSynchronize(b->upload), "hipStreamSynchronize(upload)");
 	size_t total = 0;
 	int maxSlots = 2, maxHalo = 0, schemes = 0;
 	for (auto &c : b->cs)
-		if (c.used) {
+		if (c.ready) {
 			total += (size_t) c.nBlocks;
 			maxSlots = std::max(maxSlots, c.maxSlots);
 			maxHalo = std::max(maxHalo, c.maxHalo);
@@ -160,7 +163,10 @@ void activate(e3d_batch *b) {
 	size_t n = 0;
 	for (int s = 0; s < b->nSlots; s++) {
 		const CaseHost &c = b->cs[s];
-		if (!c.used) continue;
+		if (!c.ready) {
+			b->hCases[s] = e3d::MixCase{};
+			continue;
+		}
 		for (int k = 0; k < c.nBlocks; k++) b->hBlocks[n++] = make_int2(s, k);
 		b->hCases[s] = c.desc;
 	}
@@ -172,11 +178,10 @@ void activate(e3d_batch *b) {
 	for (int s : b->statusInit) {
 		e3d::MixStatus &st = b->hStatusInit[s];
 		st = e3d::MixStatus{};
-		st.active = b->cs[s].used ? 1 : 0;
+		st.active = b->cs[s].ready ? 1 : 0;
 		hipCheck(hipMemcpyAsync(b->dStatus + s, &st, sizeof(st), hipMemcpyHostToDevice, b->stream), "H2D status");
 	}
 	b->statusInit.clear();
-	for (auto &c : b->cs) c.pending = false;
 	e3d::MixLaunch L{};
 	L.cases = b->dCases;
 	L.status = b->dStatus;
@@ -240,7 +245,6 @@ int e3d_batch_create(int device, const e3d_batch_opts *opts, e3d_batch **out) {
 		hipCheck(hipSetDevice(device), "hipSetDevice");
 		hipCheck(hipDeviceGetAttribute(&b->nCU, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
 		hipCheck(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
-		hipCheck(hipStreamCreateWithFlags(&b->upload, hipStreamNonBlocking), "hipStreamCreate");
 		b->tune.wgPerCU = o.wg_per_cu;
 		b->tune.xcds = o.xcds;
 		b->tune.nontemporal = o.nontemporal != 0;
@@ -301,26 +305,66 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 	const e3d::SolverMesh &sm = m->sm;
 	if (sm.nElem <= 0) return fail(E3D_ERR_ARG, "empty mesh");
 	if ((long long) sm.nElem + sm.nGhost >= (1LL << 31) / e3d::kRecWidth) return fail(E3D_ERR_ARG, "mesh too large");
+	// reserve a slot; the plan and the upload run without the lock so that iterations of the
+	// resident cases continue meanwhile
 	int slot = -1;
-	for (int s = 0; s < b->nSlots; s++)
-		if (!b->cs[s].used) {
-			slot = s;
-			break;
-		}
-	if (slot < 0) return fail(E3D_ERR_BUSY, "no free case slot");
-	return guarded([&] {
+	{
+		std::lock_guard<std::mutex> lk(b->mu);
+		for (int s = 0; s < b->nSlots; s++)
+			if (!b->cs[s].used) {
+				slot = s;
+				break;
+			}
+		if (slot < 0) return fail(E3D_ERR_BUSY, "no free case slot");
+		b->cs[slot].used = true;
+		b->cs[slot].ready = false;
+	}
+	double reserved = 0.0;
+	hipStream_t ust = nullptr;
+	char *mem = nullptr;
+	int rc = guarded([&] {
 		hipCheck(hipSetDevice(b->device), "hipSetDevice");
 		e3d::HostPlan P = e3d::buildPlan(sm, e3d::ORDER_MORTON, e3d::kMixBlock);
 		if (P.csrStride > 6) return fail(E3D_ERR_UNSUPPORTED, "cells with more than 6 faces are not on the device path");
 		if (e3d::mixLdsBytes((P.maxSlots + 1) & ~1, P.maxHalo) > 160 * 1024)
 			return fail(E3D_ERR_UNSUPPORTED, "a block of this mesh needs more LDS than a CU has");
+		// closure of every cell: sum over its faces of +-n S, FP64, in reference face order
+		std::vector<float> defect;
+		{
+			double worst = 0.0;
+			std::vector<double> cd(3 * (size_t) P.nElem);
+			for (int c = 0; c < P.nElem; c++) {
+				double a[3] = {0, 0, 0}, area = 0;
+				for (int j = P.fstart[c]; j < P.fstart[c + 1]; j++) {
+					const unsigned code = (unsigned) P.flist[j];
+					const int f = (int) (code & 0x7fffffffu);
+					const double sg = (code >> 31) ? -1.0 : 1.0;
+					a[0] += sg * P.fnx[f] * P.fS[f];
+					a[1] += sg * P.fny[f] * P.fS[f];
+					a[2] += sg * P.fnz[f] * P.fS[f];
+					area += P.fS[f];
+				}
+				for (int k = 0; k < 3; k++) cd[3 * (size_t) c + k] = a[k];
+				worst = std::max(worst, std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]) / area);
+			}
+			if (worst > 1e-12) {
+				defect.resize(cd.size());
+				for (size_t k = 0; k < cd.size(); k++) defect[k] = (float) cd[k];
+			}
+		}
 		CaseLayout lay;
+		lay.nDefect = defect.size() / 3;
 		initCaseHost(P, sm, *p, *co, lay);
-		if (b->used + (double) lay.total > b->budget)
-			return fail(E3D_ERR_BUSY, "HBM budget exhausted (" + std::to_string(lay.total) + " B requested)");
+		{
+			std::lock_guard<std::mutex> lk(b->mu);
+			if (b->used + (double) lay.total > b->budget)
+				return fail(E3D_ERR_BUSY, "HBM budget exhausted (" + std::to_string(lay.total) + " B requested)");
+			b->used += (double) lay.total;
+			reserved = (double) lay.total;
+		}
+		hipCheck(hipStreamCreateWithFlags(&ust, hipStreamNonBlocking), "hipStreamCreate");
 		CaseHost c;
 		c.used = true;
-		c.pending = true;
 		c.nElem = P.nElem;
 		c.nGhost = P.nGhost;
 		c.nBlocks = P.nBlocks;
@@ -330,11 +374,12 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 		fillFree(*p, c.fr);
 		c.new2old = P.new2old;
 		c.bytes = lay.total;
-		hipCheck(hipMallocAsync((void **) &c.mem, lay.total, b->upload), "hipMallocAsync");
+		hipCheck(hipMallocAsync((void **) &c.mem, lay.total, ust), "hipMallocAsync");
+		mem = c.mem;
 		char *base = c.mem;
 		auto at = [&](int i) { return base + lay.off[i]; };
 		auto up = [&](int i, const void *h, size_t bytes) {
-			if (bytes) hipCheck(hipMemcpyAsync(at(i), h, bytes, hipMemcpyHostToDevice, b->upload), "H2D case");
+			if (bytes) hipCheck(hipMemcpyAsync(at(i), h, bytes, hipMemcpyHostToDevice, ust), "H2D case");
 		};
 		const size_t nS = lay.nS, nE = lay.nE, nG = lay.nG;
 		// FP32 copies of the face geometry and cell metrics
@@ -342,7 +387,7 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 		auto upF = [&](int i, const std::vector<double> &v) {
 			for (size_t k = 0; k < v.size(); k++) f32[k] = (float) v[k];
 			up(i, f32.data(), v.size() * 4);
-			hipCheck(hipStreamSynchronize(b->upload), "hipStreamSynchronize(upload)"); // f32 is reused
+			hipCheck(hipStreamSynchronize(ust), "hipStreamSynchronize(upload)"); // f32 is reused
 		};
 		up(0, P.blockSlot.data(), P.blockSlot.size() * 4);
 		up(1, P.blockHalo.data(), P.blockHalo.size() * 4);
@@ -370,7 +415,7 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 			for (size_t i = 0; i < nE + nG; i++) rec[i * e3d::kRecWidth + e3d::R_M] = (float) q.M_inf;
 			up(15, rec.data(), rec.size() * 4);
 			up(16, rec.data(), rec.size() * 4);
-			hipCheck(hipStreamSynchronize(b->upload), "hipStreamSynchronize(upload)");
+			hipCheck(hipStreamSynchronize(ust), "hipStreamSynchronize(upload)");
 		}
 		c.nWall = (int) lay.nW;
 		c.wallCell = (int *) at(19);
@@ -379,14 +424,15 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 			c.wC[k] = (double *) at(20) + (3 + k) * std::max<size_t>(lay.nW, 1);
 		}
 		c.scalars = (double *) at(21);
+		if (!defect.empty()) up(22, defect.data(), defect.size() * 4);
 		if (lay.nW) {
 			up(19, P.wallCell.data(), lay.nW * 4);
 			for (int k = 0; k < 3; k++) {
-				hipCheck(hipMemcpyAsync(c.wN[k], P.wallN[k].data(), lay.nW * 8, hipMemcpyHostToDevice, b->upload), "H2D");
-				hipCheck(hipMemcpyAsync(c.wC[k], P.wallC[k].data(), lay.nW * 8, hipMemcpyHostToDevice, b->upload), "H2D");
+				hipCheck(hipMemcpyAsync(c.wN[k], P.wallN[k].data(), lay.nW * 8, hipMemcpyHostToDevice, ust), "H2D");
+				hipCheck(hipMemcpyAsync(c.wC[k], P.wallC[k].data(), lay.nW * 8, hipMemcpyHostToDevice, ust), "H2D");
 			}
 		}
-		hipCheck(hipStreamSynchronize(b->upload), "hipStreamSynchronize(upload)");
+		hipCheck(hipStreamSynchronize(ust), "hipStreamSynchronize(upload)");
 		double uinf2 = q.u_inf * q.u_inf + q.v_inf * q.v_inf + q.w_inf * q.w_inf;
 		c.cpFactor = 1 / (0.5 * q.rho_inf * uinf2);
 
@@ -415,6 +461,7 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 		d.sx = (const float *) at(11);
 		d.sy = (const float *) at(12);
 		d.sz = (const float *) at(13);
+		d.defect = defect.empty() ? nullptr : (const float *) at(22);
 		d.W = (double *) at(14);
 		d.rec[0] = (float *) at(15);
 		d.rec[1] = (float *) at(16);
@@ -436,18 +483,36 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 		d.wInfD = q.w_inf;
 		d.pInfD = q.p_inf;
 		d.hInfD = q.H_inf;
-		b->used += (double) lay.total;
-		b->cs[slot] = std::move(c);
-		b->statusInit.push_back(slot);
-		b->dirty = true;
+		c.ready = true;
+		{
+			std::lock_guard<std::mutex> lk(b->mu);
+			b->cs[slot] = std::move(c);
+			b->statusInit.push_back(slot);
+			b->dirty = true;
+		}
+		mem = nullptr;
 		*case_id = slot;
 		return E3D_OK;
 	});
+	if (ust) {
+		if (mem) {
+			(void) hipFreeAsync(mem, ust);
+			(void) hipStreamSynchronize(ust);
+		}
+		(void) hipStreamDestroy(ust);
+	}
+	if (rc != E3D_OK) {
+		std::lock_guard<std::mutex> lk(b->mu);
+		b->used -= reserved;
+		b->cs[slot] = CaseHost{};
+	}
+	return rc;
 }
 
 int e3d_batch_run(e3d_batch *b, int n) {
 	if (!b || n < 0) return fail(E3D_ERR_ARG, "bad argument");
 	return guarded([&] {
+		std::lock_guard<std::mutex> lk(b->mu);
 		hipCheck(hipSetDevice(b->device), "hipSetDevice");
 		activate(b);
 		if (b->L.grid == 0 || n == 0) return E3D_OK;
@@ -486,6 +551,7 @@ int e3d_batch_run(e3d_batch *b, int n) {
 int e3d_batch_wait(e3d_batch *b) {
 	if (!b) return fail(E3D_ERR_ARG, "null batch");
 	return guarded([&] {
+		std::lock_guard<std::mutex> lk(b->mu);
 		hipCheck(hipSetDevice(b->device), "hipSetDevice");
 		b->sync();
 		return E3D_OK;
@@ -493,8 +559,10 @@ int e3d_batch_wait(e3d_batch *b) {
 }
 
 int e3d_batch_status(e3d_batch *b, int id, int *done, int *iters, double *rms0, double *rmsLast) {
-	if (!b || id < 0 || id >= b->nSlots || !b->cs[id].used) return fail(E3D_ERR_ARG, "no such case");
+	if (!b || id < 0 || id >= b->nSlots) return fail(E3D_ERR_ARG, "no such case");
 	return guarded([&] {
+		std::lock_guard<std::mutex> lk(b->mu);
+		if (!b->cs[id].ready) return fail(E3D_ERR_ARG, "no such case");
 		hipCheck(hipSetDevice(b->device), "hipSetDevice");
 		activate(b);
 		e3d::MixStatus *h = b->hStatus + id;
@@ -508,10 +576,38 @@ int e3d_batch_status(e3d_batch *b, int id, int *done, int *iters, double *rms0, 
 	});
 }
 
+int e3d_batch_poll(e3d_batch *b, int *ids, int *done, int *iters, double *rmsLast, int cap, int *n) {
+	if (!b || !n) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		std::lock_guard<std::mutex> lk(b->mu);
+		hipCheck(hipSetDevice(b->device), "hipSetDevice");
+		activate(b);
+		hipCheck(hipMemcpyAsync(b->hStatus, b->dStatus, sizeof(e3d::MixStatus) * b->nSlots, hipMemcpyDeviceToHost,
+		                        b->stream),
+		         "D2H status");
+		b->sync();
+		int k = 0;
+		for (int s = 0; s < b->nSlots; s++) {
+			if (!b->cs[s].ready) continue;
+			if (k < cap) {
+				if (ids) ids[k] = s;
+				if (done) done[k] = b->hStatus[s].done;
+				if (iters) iters[k] = b->hStatus[s].iter;
+				if (rmsLast) rmsLast[k] = b->hStatus[s].rmsLast;
+			}
+			k++;
+		}
+		*n = k;
+		return E3D_OK;
+	});
+}
+
 int e3d_batch_result(e3d_batch *b, int id, double *hist, int cap, int *nHist, double coeffs[3], double cfOut[3],
                      double cmOut[3]) {
-	if (!b || id < 0 || id >= b->nSlots || !b->cs[id].used) return fail(E3D_ERR_ARG, "no such case");
+	if (!b || id < 0 || id >= b->nSlots) return fail(E3D_ERR_ARG, "no such case");
 	return guarded([&] {
+		std::lock_guard<std::mutex> lk(b->mu);
+		if (!b->cs[id].ready) return fail(E3D_ERR_ARG, "no such case");
 		hipCheck(hipSetDevice(b->device), "hipSetDevice");
 		activate(b);
 		CaseHost &c = b->cs[id];
@@ -550,8 +646,10 @@ int e3d_batch_result(e3d_batch *b, int id, double *hist, int cap, int *nHist, do
 }
 
 int e3d_batch_get_state(e3d_batch *b, int id, double *const fields[8]) {
-	if (!b || !fields || id < 0 || id >= b->nSlots || !b->cs[id].used) return fail(E3D_ERR_ARG, "no such case");
+	if (!b || !fields || id < 0 || id >= b->nSlots) return fail(E3D_ERR_ARG, "no such case");
 	return guarded([&] {
+		std::lock_guard<std::mutex> lk(b->mu);
+		if (!b->cs[id].ready) return fail(E3D_ERR_ARG, "no such case");
 		hipCheck(hipSetDevice(b->device), "hipSetDevice");
 		activate(b);
 		CaseHost &c = b->cs[id];
@@ -589,19 +687,31 @@ int e3d_batch_get_state(e3d_batch *b, int id, double *const fields[8]) {
 }
 
 int e3d_batch_remove(e3d_batch *b, int id) {
-	if (!b || id < 0 || id >= b->nSlots || !b->cs[id].used) return fail(E3D_ERR_ARG, "no such case");
+	if (!b || id < 0 || id >= b->nSlots) return fail(E3D_ERR_ARG, "no such case");
 	return guarded([&] {
+		std::lock_guard<std::mutex> lk(b->mu);
+		if (!b->cs[id].ready) return fail(E3D_ERR_ARG, "no such case");
 		hipCheck(hipSetDevice(b->device), "hipSetDevice");
 		CaseHost &c = b->cs[id];
 		// kernels already queued may still read the case; the free is ordered after them
 		hipCheck(hipFreeAsync(c.mem, b->stream), "hipFreeAsync");
 		b->used -= (double) c.bytes;
 		c = CaseHost{};
-		b->hCases[id] = e3d::MixCase{};
 		b->statusInit.push_back(id);
 		b->dirty = true;
 		return E3D_OK;
 	});
+}
+
+int e3d_batch_set_tuning(e3d_batch *b, int wgPerCU, int xcds, int nontemporal) {
+	if (!b) return fail(E3D_ERR_ARG, "null batch");
+	if (wgPerCU <= 0 || wgPerCU > 32 || xcds <= 0 || xcds > 64) return fail(E3D_ERR_ARG, "invalid launch shape");
+	b->tune.wgPerCU = wgPerCU;
+	b->tune.xcds = xcds;
+	b->tune.nontemporal = nontemporal != 0;
+	b->dirty = true; // recompute the grid; the graph is captured again when the launch changes
+	freeGraph(b);
+	return E3D_OK;
 }
 
 int e3d_batch_info(e3d_batch *b, int *nCases, double *bytes, double *budget) {
@@ -624,6 +734,7 @@ int e3d_batch_set_timing(e3d_batch *b, int enable) {
 int e3d_batch_timing(e3d_batch *b, double *avgMs, int *launches, int *perIteration) {
 	if (!b) return fail(E3D_ERR_ARG, "null batch");
 	return guarded([&] {
+		std::lock_guard<std::mutex> lk(b->mu);
 		b->sync();
 		double total = 0;
 		int n = (int) (b->evUsed / 2);
@@ -644,7 +755,6 @@ void e3d_batch_destroy(e3d_batch *b) {
 	if (!b) return;
 	(void) hipSetDevice(b->device);
 	(void) hipStreamSynchronize(b->stream);
-	(void) hipStreamSynchronize(b->upload);
 	freeGraph(b);
 	for (auto &c : b->cs)
 		if (c.used && c.mem) (void) hipFreeAsync(c.mem, b->stream);
@@ -660,7 +770,6 @@ void e3d_batch_destroy(e3d_batch *b) {
 	if (b->hBlocks) (void) hipHostFree(b->hBlocks);
 	for (auto e : b->evPool) (void) hipEventDestroy(e);
 	(void) hipStreamDestroy(b->stream);
-	(void) hipStreamDestroy(b->upload);
 	delete b;
 }
 

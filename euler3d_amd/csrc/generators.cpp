@@ -362,8 +362,10 @@ MeshFile genWing(int ni, int nj, int nk, double t, double sweepDeg, double taper
 	// chordwise coordinate xi in [0,1] on the wing (cosine clustering at LE and TE)
 	std::vector<double> xi(nChord + 1);
 	for (int q = 0; q <= nChord; q++) xi[q] = 0.5 * (1.0 - std::cos(pi * q / nChord));
-	// vertical distribution from the sheet to +-farfield
-	std::vector<double> etaY = geometricEta(0.02 * 1.0 / 1.0 * 0.25, farfield, jw);
+	// vertical distribution from the sheet to +-farfield: jw cells below, nj - 1 - jw above (one
+	// more when nj is even)
+	std::vector<double> etaLo = geometricEta(0.02 * 1.0 / 1.0 * 0.25, farfield, jw);
+	std::vector<double> etaHi = geometricEta(0.02 * 1.0 / 1.0 * 0.25, farfield, nj - 1 - jw);
 	auto xAt = [&](int i, double z) {
 		double x0 = xle(z), c = chord(z);
 		if (i >= iLE && i <= iTE) return x0 + c * xi[i - iLE];
@@ -408,8 +410,8 @@ MeshFile genWing(int ni, int nj, int nk, double t, double sweepDeg, double taper
 			double ht = halfThick(i, k);
 			for (int j = 0; j < nj; j++) {
 				double y;
-				if (j < jw) y = -ht - etaY[jw - j] * (farfield - ht);
-				else if (j > jw) y = ht + etaY[j - jw] * (farfield - ht);
+				if (j < jw) y = -ht - etaLo[jw - j] * (farfield - ht);
+				else if (j > jw) y = ht + etaHi[j - jw] * (farfield - ht);
 				else y = -ht; // lower copy (or the single sheet node)
 				size_t n = base(i, j, k);
 				m.xyz[3 * n] = x;

@@ -182,14 +182,18 @@ struct GhostD {
 };
 
 // Wall (PhysicalBC.cpp:124-162) and symmetry (:165-231) ghost: mirrored velocity, same rho and p.
-// u_g - u_inf = du_i - 2 V_i n; H is unchanged by the mirror.
+// u_g - u_inf = du_i - 2 V_i n. The face "unit" normal of a non-planar quad is N / area with
+// |n| < 1 (Metrics.cpp:139), so the mirror changes |u|: |u_g|^2 = |u_i|^2 + 4 V^2 (|n|^2 - 1),
+// and with it E and H (H_g - H_i = 2 V^2 (|n|^2 - 1)).
 __device__ __forceinline__ GhostD mirrorD(const D &I, float nx, float ny, float nz, const Free32 &f) {
 	const float rho = f.r + I.r, p = f.p + I.p;
 	const float ui = f.u + I.u, vi = f.v + I.v, wi = f.w + I.w;
 	const float V = ui * nx + vi * ny + wi * nz;
 	const float c = fsqrt(f.g * p * frcp(rho));
-	const float M = fsqrt(ui * ui + vi * vi + wi * wi) * frcp(c);
-	return {{I.r, I.u - 2.0f * V * nx, I.v - 2.0f * V * ny, I.w - 2.0f * V * nz, I.p, I.H}, M};
+	const float dn = 2.0f * V * V * ((nx * nx + ny * ny + nz * nz) - 1.0f);
+	const float q2 = fmaxf(ui * ui + vi * vi + wi * wi + 2.0f * dn, 0.0f);
+	const float M = fsqrt(q2) * frcp(c);
+	return {{I.r, I.u - 2.0f * V * nx, I.v - 2.0f * V * ny, I.w - 2.0f * V * nz, I.p, I.H + dn}, M};
 }
 
 // Farfield ghost (EulerSolver.cpp:205-226 branch on the ghost's previous state, then
@@ -349,7 +353,24 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 					F[3] = G.d.p * nz - minf * f.w;
 					F[4] = -minf * f.h;
 				} else if (kind == 2) {
-					fluxDelta(SCHEME, Lc, G.d, nx, ny, nz, f, F);
+					const float n2 = nx * nx + ny * ny + nz * nz;
+					if (SCHEME == 0 && fabsf(n2 - 1.0f) < 1e-5f) {
+						// Roe flux between a state and its mirror image through a unit normal, in
+						// closed form: the Roe-averaged normal velocity vanishes, so only the two
+						// acoustic waves remain and F = (0, (p + rho V^2 + rho c~ V) n, 0) with
+						// c~^2 = c^2 + (gamma - 1) V^2 / 2 (c~ > c_L / 15: no entropy fix)
+						const float rho = f.r + Lc.r;
+						const float V = (f.u + Lc.u) * nx + (f.v + Lc.v) * ny + (f.w + Lc.w) * nz;
+						const float ct = fsqrt(f.g * (f.p + Lc.p) * frcp(rho) + 0.5f * (f.g - 1.0f) * V * V);
+						const float q = Lc.p + rho * V * (V + ct);
+						F[0] = -minf;
+						F[1] = q * nx - minf * f.u;
+						F[2] = q * ny - minf * f.v;
+						F[3] = q * nz - minf * f.w;
+						F[4] = -minf * f.h;
+					} else {
+						fluxDelta(SCHEME, Lc, G.d, nx, ny, nz, f, F);
+					}
 				} else {
 					// farfield: physical flux of the ghost state (Solver::Fc)
 					physDelta(G.d, nx, ny, nz, Vinf, minf, f, F);
@@ -391,6 +412,17 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 					r[k] = tt;
 				}
 			}
+			if (C.defect) {
+				// F_inf through the cell's unclosed remainder (exact freestream flux, FP64)
+				const double cx = C.defect[3 * (size_t) e], cy = C.defect[3 * (size_t) e + 1], cz = C.defect[3 * (size_t) e + 2];
+				const double V = C.uInfD * cx + C.vInfD * cy + C.wInfD * cz;
+				const double m = C.rInfD * V;
+				r[0] += m;
+				r[1] += m * C.uInfD + C.pInfD * cx;
+				r[2] += m * C.vInfD + C.pInfD * cy;
+				r[3] += m * C.wInfD + C.pInfD * cz;
+				r[4] += m * C.hInfD;
+			}
 			// TimeStepMethod2 (TimeIntegration.cpp:8-40) on the FP32 state
 			const float rho = f.r + ra.x, u = f.u + ra.y, v = f.v + ra.z, w = f.w + ra.w, p = f.p + rb.x;
 			const float c = fsqrt(f.g * p * frcp(rho));
@@ -410,7 +442,7 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 			const D d = {(float) (nw[0] - C.rInfD), (float) (nu - C.uInfD), (float) (nv - C.vInfD), (float) (nwv - C.wInfD),
 			             (float) (np - C.pInfD), (float) (nH - C.hInfD)};
 			// stored Mach number c / |u| as FlowField::updateWRungeKutta keeps it (FlowField.cpp:157)
-			const float M = (float) (sqrt(gd * (np / nw[0])) / sqrt(q2));
+			const float M = fsqrt(f.g * (float) np * frcp((float) nw[0])) * __builtin_amdgcn_rsqf((float) q2);
 			storeRec(out, e, d, M);
 			if (stage == 4) {
 #pragma unroll
@@ -428,7 +460,7 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 
 // one workgroup per case slot: RMS of the last stage (fixed-order sum of the block partials),
 // residual history, convergence and buffer parity
-constexpr int kEndThreads = 256;
+constexpr int kEndThreads = 1024;
 __global__ void __launch_bounds__(kEndThreads) kMixIterEnd(const MixCase *cases, MixStatus *status) {
 	__shared__ double red[kEndThreads];
 	const int slot = blockIdx.x;

@@ -48,6 +48,9 @@ struct MixCase {
 	const uint16_t *csr;
 	// cell metrics (FP32) and volume
 	const float *vol, *sx, *sy, *sz;
+	// closure defect sum(+-n S) of each cell (3 floats), null when every cell is closed: the
+	// perturbation form adds F_inf . defect back so that unclosed cells keep the reference's residual
+	const float *defect;
 	// state
 	double *W;      // 5 x nElem conservative state (rho, rho u, rho v, rho w, rho E), FP64
 	float *rec[2];  // (nElem + nGhost) x kRecWidth FP32 records, ping-pong

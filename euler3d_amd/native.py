@@ -93,10 +93,12 @@ _SIGS = {
     "e3d_batch_run": (C.c_int, [P, C.c_int]),
     "e3d_batch_wait": (C.c_int, [P]),
     "e3d_batch_status": (C.c_int, [P, C.c_int, IP, IP, DP, DP]),
+    "e3d_batch_poll": (C.c_int, [P, IP, IP, IP, DP, C.c_int, IP]),
     "e3d_batch_result": (C.c_int, [P, C.c_int, DP, C.c_int, IP, DP, DP, DP]),
     "e3d_batch_get_state": (C.c_int, [P, C.c_int, C.POINTER(DP)]),
     "e3d_batch_remove": (C.c_int, [P, C.c_int]),
     "e3d_batch_info": (C.c_int, [P, IP, DP, DP]),
+    "e3d_batch_set_tuning": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
     "e3d_batch_set_timing": (C.c_int, [P, C.c_int]),
     "e3d_batch_timing": (C.c_int, [P, DP, IP, IP]),
     "e3d_batch_destroy": (None, [P]),

@@ -292,6 +292,20 @@ class Batch:
         N.check(N.lib().e3d_batch_status(self._h, int(cid), C.byref(done), C.byref(it), C.byref(r0), C.byref(r1)))
         return {"done": bool(done.value), "iterations": it.value, "rms0": r0.value, "rms": r1.value}
 
+    def poll(self) -> list[dict]:
+        """Wait for the queued iterations; progress of every resident case."""
+        cap = 1024
+        while True:
+            ids, done, it = (np.empty(cap, dtype=np.int32) for _ in range(3))
+            rms = np.empty(cap)
+            n = C.c_int()
+            N.check(N.lib().e3d_batch_poll(self._h, N.iptr(ids), N.iptr(done), N.iptr(it), N.dptr(rms), cap,
+                                           C.byref(n)))
+            if n.value <= cap:
+                return [{"id": int(ids[k]), "done": bool(done[k]), "iterations": int(it[k]), "rms": float(rms[k])}
+                        for k in range(n.value)]
+            cap = n.value
+
     def result(self, cid: int) -> dict:
         cap = self._n[cid][2]
         hist = np.empty(max(cap, 1))
@@ -312,6 +326,9 @@ class Batch:
     def remove(self, cid: int) -> None:
         N.check(N.lib().e3d_batch_remove(self._h, int(cid)))
         self._n.pop(cid, None)
+
+    def set_tuning(self, wg_per_cu: int, xcds: int, nontemporal: bool) -> None:
+        N.check(N.lib().e3d_batch_set_tuning(self._h, int(wg_per_cu), int(xcds), int(nontemporal)))
 
     def info(self) -> dict:
         n, used, budget = C.c_int(), C.c_double(), C.c_double()

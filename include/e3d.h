@@ -179,7 +179,9 @@ int e3d_batch_create(int device, const e3d_batch_opts *opts, e3d_batch **out);
 int e3d_batch_case_bytes(const e3d_mesh *m, const e3d_case_opts *co, double *bytes);
 /* Admits a case (freestream start, FlowField::Initialize) and returns its id. The mesh may be
  * freed afterwards. E3D_ERR_BUSY when no slot is free or the case would exceed the budget. The
- * upload overlaps the iterations already queued; the case starts with the next e3d_batch_run. */
+ * upload overlaps the iterations already queued; the case starts with the next e3d_batch_run. May
+ * be called from other threads while e3d_batch_run / e3d_batch_poll run (a batch is otherwise
+ * used from one thread). */
 int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3d_case_opts *co, int *case_id);
 /* Queues n iterations of every resident, unfinished case and returns without waiting. */
 int e3d_batch_run(e3d_batch *b, int n);
@@ -188,6 +190,8 @@ int e3d_batch_wait(e3d_batch *b);
 /* Waits, then reports a case's progress: done (converged or out of iterations), iterations,
  * RMS of the first and of the last iteration. */
 int e3d_batch_status(e3d_batch *b, int case_id, int *done, int *iters, double *rms0, double *rms_last);
+/* Waits, then lists the resident cases (up to cap) with their progress; *n receives their count. */
+int e3d_batch_poll(e3d_batch *b, int *ids, int *done, int *iters, double *rms_last, int cap, int *n);
 /* Residual RMS history (one value per iteration, at most cap) and the force and moment
  * coefficients of the current state: coeffs = CL, CD, CM (AeroCoefficients::Update,
  * src/solver/AeroCoefficients.cpp:38-52, FP64). */
@@ -198,6 +202,9 @@ int e3d_batch_result(e3d_batch *b, int case_id, double *hist, int cap, int *n_hi
 int e3d_batch_get_state(e3d_batch *b, int case_id, double *const fields[8]);
 /* Retires a case and releases its device memory (stream-ordered; no device-wide wait). */
 int e3d_batch_remove(e3d_batch *b, int case_id);
+/* Changes the launch shape (persistent workgroups per CU, XCD order, non-temporal loads) of the
+ * next iterations; results are bitwise the same for every shape. Used by the autotuner. */
+int e3d_batch_set_tuning(e3d_batch *b, int wg_per_cu, int xcds, int nontemporal);
 /* resident cases, bytes they hold, budget */
 int e3d_batch_info(e3d_batch *b, int *n_cases, double *bytes, double *budget);
 /* Per-launch timing of the stage kernel over the next e3d_batch_run calls (HIP events around

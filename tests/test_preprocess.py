@@ -177,3 +177,25 @@ def test_unknown_boundary_tag_rejected():
     m = Mesh.box(2, 2, 2, tags=("farfield", "farfield", "wall", "wall", "sym1", "sym"))
     with pytest.raises(E3DError, match="sym1"):
         m.preprocess(1.0)
+
+
+@pytest.mark.parametrize("dims", [(24, 12, 9), (25, 13, 9), (30, 16, 11), (33, 17, 12)])
+def test_wing_generator_cells_are_closed(dims):
+    """Every generated wing cell is closed (sum of n S over its faces vanishes) and every node lies
+    in the box, for odd and even node counts."""
+    ni, nj, nk = dims
+    m = Mesh.wing(ni=ni, nj=nj, nk=nk).preprocess(1.0)
+    info = m.info()
+    n = info["n_elem"]
+    xyz = m.get("xyz").reshape(-1, 3)
+    assert np.all(np.isfinite(xyz)) and np.all(np.abs(xyz[:, 1]) <= 15.0 + 1e-12)
+    f2e = m.get("face2element").reshape(-1, 2)
+    v = m.get("faceNormalsUnit").reshape(-1, 3) * m.get("faceSurfaces")[:, None]
+    C = np.zeros((n, 3))
+    A = np.zeros(n)
+    np.add.at(C, f2e[:, 0], v)
+    inner = f2e[:, 1] < n
+    np.subtract.at(C, f2e[inner, 1], v[inner])
+    np.add.at(A, f2e[:, 0], m.get("faceSurfaces"))
+    assert np.max(np.linalg.norm(C, axis=1) / A) < 1e-12
+    assert np.all(m.get("cellVolumes") > 0)
