@@ -44,6 +44,8 @@ int guarded(F &&f) {
 		return fail(E3D_ERR_DEVICE, e.what());
 	} catch (const std::invalid_argument &e) {
 		return fail(E3D_ERR_ARG, e.what());
+	} catch (const std::domain_error &e) {
+		return fail(E3D_ERR_NUMERIC, e.what());
 	} catch (const std::exception &e) {
 		return fail(E3D_ERR_MESH, e.what());
 	}

@@ -301,6 +301,9 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 	if (!m->preprocessed) return fail(E3D_ERR_ARG, "mesh is not preprocessed (call e3d_mesh_preprocess)");
 	if (p->scheme != 0 && p->scheme != 1) return fail(E3D_ERR_ARG, "SCHEME must be 0 (Roe) or 1 (AUSM)");
 	if (!(p->cfl > 0) || !(p->gamma > 1)) return fail(E3D_ERR_ARG, "invalid CFL or GAMMA");
+	if (p->time_integration != 2 || p->residual_smoothing)
+		return fail(E3D_ERR_UNSUPPORTED, "the mixed-precision engine runs RK5 (TIME_INTEGRATION= 2) without residual "
+		                                 "smoothing; use e3d_create for the other integrators");
 	if (co->max_iter <= 0 || co->hist_cap < 0 || co->drop < 0) return fail(E3D_ERR_ARG, "invalid case options");
 	const e3d::SolverMesh &sm = m->sm;
 	if (sm.nElem <= 0) return fail(E3D_ERR_ARG, "empty mesh");

@@ -85,6 +85,13 @@ struct e3d_ctx {
 	DBuf<double> st[2];
 	int cur = 0;
 	DBuf<double> W0, fs, partial, scalars;
+	// residual smoothing (RESIDUAL_SMOOTHING= 1): neighbour lists in device numbering, residual
+	// before/during/after the sweeps, per-block largest change, decision and NaN flags
+	int nbrStride = 0;
+	DBuf<int> nbr;
+	DBuf<uint8_t> nface;
+	DBuf<double> R0, R1, R2, blockMax;
+	DBuf<int> smoothFlag;
 	// aero (wall list order)
 	int nWall = 0;
 	DBuf<int> wallCell;
@@ -107,6 +114,13 @@ void initFreestream(e3d_ctx *c) {
 	for (size_t i = 0; i < (size_t) c->nTot; i++)
 		for (int k = 0; k < 8; k++) h[i * e3d::kStateWidth + k] = vals[k];
 	c->st[0].upload(h.data(), h.size(), c->stream);
+	// FlowField::Initialize (FlowField.cpp:83-95): rho u, rho v, rho w, rho E at rho_inf * (.)_inf,
+	// the explicit Euler step's conservative state
+	std::vector<double> w(5 * (size_t) c->nElem);
+	const double wi[5] = {c->fr.rho_inf, c->fr.rho_inf * c->fr.u_inf, c->fr.rho_inf * c->fr.v_inf,
+	                      c->fr.rho_inf * c->fr.w_inf, c->fr.rho_inf * c->fr.E_inf};
+	for (int k = 0; k < 5; k++) std::fill(w.begin() + k * (size_t) c->nElem, w.begin() + (k + 1) * (size_t) c->nElem, wi[k]);
+	c->W0.upload(w.data(), w.size(), c->stream);
 	c->sync();
 	c->machDerived = 0;
 }
@@ -129,24 +143,51 @@ void runStage(e3d_ctx *c, int stage, double *rmsPartial) {
 		evA = c->evPool[c->evUsed++];
 		evB = c->evPool[c->evUsed++];
 	}
-	if (c->fused) {
+	const int ti = c->prm.time_integration;
+	const bool smooth = c->prm.residual_smoothing != 0;
+	if (c->fused && !smooth && ti != 1) {
+		// one fused launch: ghosts, fluxes, gather, dt, update (RK stage or explicit Euler step)
 		auto o = c->state(c->cur ^ 1);
 		record(evA);
-		e3d::launchStageFused(c->dm, c->plan, s, o, c->fr, c->prm.scheme, c->W0.p, alphas[stage], stage == 0,
-		                      c->machDerived, rmsPartial, c->tune, c->stream);
+		e3d::launchStageFused(c->dm, c->plan, s, o, c->fr, c->prm.scheme, ti == 0 ? 1 : 0, c->W0.p, alphas[stage],
+		                      stage == 0, c->machDerived, rmsPartial, c->tune, c->stream);
 		record(evB);
 		c->cur ^= 1;
 	} else {
+		// EulerSolver::Run / RungeKutta body: updateBC, computeResidual, smoothResiduals, update in place
 		record(evA);
 		e3d::launchBc(c->dm, s, c->fr, c->machDerived, c->stream);
 		e3d::launchFlux(c->dm, s, c->fr, c->prm.scheme, c->fs.p, nullptr, c->stream);
-		e3d::launchStage(c->dm, s, c->fr, c->fs.p, c->W0.p, alphas[stage], stage == 0, rmsPartial, nullptr, c->stream);
+		const double *Rin = nullptr;
+		if (smooth) {
+			e3d::launchGather(c->dm, c->fs.p, c->R0.p, c->stream);
+			e3d::launchSmooth(c->R0.p, c->R1.p, c->R2.p, c->nbr.p, c->nface.p, c->nbrStride, c->nElem, c->blockMax.p,
+			                  c->smoothFlag.p, c->stream);
+			Rin = c->R2.p;
+		}
+		const int integ = ti == 2 ? 0 : ti == 0 ? 1 : 2;
+		e3d::launchStage(c->dm, s, c->fr, c->fs.p, Rin, c->W0.p, alphas[stage], stage == 0, integ, rmsPartial, nullptr,
+		                 c->stream);
 		record(evB);
 	}
-	c->machDerived = 1;
+	// the stored interior Mach number becomes c/|u| with the first update; TIME_INTEGRATION= 1
+	// never updates the state
+	if (ti != 1) c->machDerived = 1;
 }
 
-const char *stageKernelName(const e3d_ctx *c) { return c->fused ? "kStageFused" : "kBc+kFlux+kStage"; }
+const char *stageKernelName(const e3d_ctx *c) {
+	if (c->fused && !c->prm.residual_smoothing && c->prm.time_integration != 1) return "kStageFused";
+	return c->prm.residual_smoothing ? "kBc+kFlux+kGather+kSmooth+kStage" : "kBc+kFlux+kStage";
+}
+
+// NaN in the residual smoothing (the reference exits the process, ResidualVar.hpp:96-104)
+void checkSmoothing(e3d_ctx *c) {
+	if (!c->prm.residual_smoothing) return;
+	int f[2] = {0, 0};
+	hipCheck(hipMemcpyAsync(f, c->smoothFlag.p, sizeof(f), hipMemcpyDeviceToHost, c->stream), "D2H flag");
+	c->sync();
+	if (f[1]) throw std::domain_error("NaN residual in the residual smoothing");
+}
 
 // host reference-order field k <-> device state (interior permuted, ghosts as they are, 8 doubles
 // per cell)
@@ -161,7 +202,11 @@ void toReference(const e3d_ctx *c, const std::vector<double> &dev, int k, double
 
 // one iteration of Run's body; the sum of R_rho^2 of the last stage lands in *dsum (device)
 void runIteration(e3d_ctx *c, double *dsum) {
-	for (int stage = 0; stage < 5; stage++) runStage(c, stage, stage == 4 ? c->partial.p : nullptr);
+	if (c->prm.time_integration == 2) {
+		for (int stage = 0; stage < 5; stage++) runStage(c, stage, stage == 4 ? c->partial.p : nullptr);
+	} else {
+		runStage(c, 0, c->partial.p); // explicit Euler step, or (implicit) residual only
+	}
 	e3d::launchSum(c->partial.p, e3d::stageBlocks(c->nElem), dsum, c->stream);
 }
 
@@ -385,9 +430,8 @@ void e3d_mesh_free(e3d_mesh *m) { delete m; }
 int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out) {
 	if (!m || !p || !out) return fail(E3D_ERR_ARG, "null argument");
 	if (!m->preprocessed) return fail(E3D_ERR_ARG, "mesh is not preprocessed (call e3d_mesh_preprocess)");
-	if (p->time_integration != 2)
-		return fail(E3D_ERR_UNSUPPORTED, "only TIME_INTEGRATION= 2 (5-stage Runge-Kutta) is on the device path");
-	if (p->residual_smoothing) return fail(E3D_ERR_UNSUPPORTED, "RESIDUAL_SMOOTHING is not on the device path");
+	if (p->time_integration < 0 || p->time_integration > 2)
+		return fail(E3D_ERR_ARG, "TIME_INTEGRATION must be 0 (explicit Euler), 1 (implicit Euler) or 2 (RK5)");
 	if (p->scheme != 0 && p->scheme != 1) return fail(E3D_ERR_ARG, "SCHEME must be 0 (Roe) or 1 (AUSM)");
 	if (!(p->cfl > 0) || !(p->sampling > 0) || p->max_iter < 0) return fail(E3D_ERR_ARG, "invalid CFL/SAMPLING/MAX_ITER");
 	const e3d::SolverMesh &sm = m->sm;
@@ -472,6 +516,33 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 
 		for (auto &b : c->st) b.alloc((size_t) c->nTot * e3d::kStateWidth);
 		c->W0.alloc(5 * nE);
+		if (p->residual_smoothing) {
+			// neighbour of each local face in device numbering (EulerSolver.cpp:495-503 walks
+			// GetElement2ElementID in this order); -2 marks a ghost, -1 pads
+			int stride = 0;
+			for (int e = 0; e < sm.nElem; e++)
+				stride = std::max(stride, sm.element2faceStart[e + 1] - sm.element2faceStart[e]);
+			std::vector<int> nb((size_t) nE * stride, -1);
+			std::vector<uint8_t> nf(nE);
+			for (int d = 0; d < sm.nElem; d++) {
+				int old = P.new2old[d];
+				int b = sm.element2faceStart[old], en = sm.element2faceStart[old + 1];
+				nf[d] = (uint8_t) (en - b);
+				for (int j = b; j < en; j++) {
+					int s2 = sm.element2element[j];
+					nb[(size_t) d * stride + (j - b)] = s2 < sm.nElem ? P.old2new[s2] : -2;
+				}
+			}
+			c->nbrStride = stride;
+			c->nbr.alloc(nb.size()), c->nbr.upload(nb.data(), nb.size(), s);
+			c->nface.alloc(nE), c->nface.upload(nf.data(), nE, s);
+			for (auto *b : {&c->R0, &c->R1, &c->R2}) b->alloc(5 * nE);
+			c->blockMax.alloc(std::max(1, e3d::stageBlocks(c->nElem)));
+			c->smoothFlag.alloc(2);
+			int zero[2] = {0, 0};
+			c->smoothFlag.upload(zero, 2, s);
+			c->sync();
+		}
 		c->fs.alloc(5 * nF);
 		c->partial.alloc(std::max(1, e3d::stageBlocks(c->nElem)));
 		c->scalars.alloc(16);
@@ -498,6 +569,15 @@ int e3d_set_state(e3d_ctx *c, const double *const fields[8]) {
 		std::vector<double> h((size_t) c->nTot * e3d::kStateWidth);
 		for (int k = 0; k < 8; k++) toDevice(c, fields[k], k, h);
 		c->st[c->cur].upload(h.data(), h.size(), c->stream);
+		std::vector<double> w(5 * (size_t) c->nElem);
+		const size_t n = (size_t) c->nElem;
+		for (size_t i = 0; i < n; i++) {
+			const double *cell = &h[i * e3d::kStateWidth];
+			const double rho = cell[e3d::S_RHO];
+			w[i] = rho, w[n + i] = cell[e3d::S_U] * rho, w[2 * n + i] = cell[e3d::S_V] * rho;
+			w[3 * n + i] = cell[e3d::S_W] * rho, w[4 * n + i] = cell[e3d::S_E] * rho;
+		}
+		c->W0.upload(w.data(), w.size(), c->stream);
 		c->sync();
 		c->machDerived = 0;
 		return E3D_OK;
@@ -611,6 +691,7 @@ int e3d_iterate(e3d_ctx *c, int n, double *rms_sum) {
 		hipCheck(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
 		c->check();
 		c->sync();
+		checkSmoothing(c);
 		float ms = 0;
 		hipCheck(hipEventElapsedTime(&ms, c->ev0, c->ev1), "hipEventElapsedTime");
 		c->lastMs = ms;
@@ -667,6 +748,7 @@ int e3d_run(e3d_ctx *c, double *rows, int max_rows, int *n_rows, int *iters, dou
 		while (nb < q.max_iter) {
 			runIteration(c, dsum);
 			c->check();
+			checkSmoothing(c);
 			double sum = 0;
 			hipCheck(hipMemcpyAsync(&sum, dsum, sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
 			c->sync();

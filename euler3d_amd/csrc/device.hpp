@@ -81,11 +81,16 @@ void launchFlux(const DevMesh &m, DevState s, const DevFree &q, int scheme, doub
 // gather-only: R[5][nElem] from fs (parity hook)
 void launchGather(const DevMesh &m, const double *fs, double *R, hipStream_t st);
 void launchTimestep(const DevMesh &m, DevState s, const DevFree &q, double *dt, hipStream_t st);
-// Unfused RK stage (needs launchBc + launchFlux before it): gather residual, local dt,
-// W_k = W0 - alpha*dt*R/V, primitive update in place. firstStage also stores W0; when
-// rmsPartial != nullptr each block writes its partial sum of R_rho^2.
-void launchStage(const DevMesh &m, DevState s, const DevFree &q, const double *fs, double *W0, double alpha,
-                 int firstStage, double *rmsPartial, double *Rout, hipStream_t st);
+// Unfused stage (needs launchBc + launchFlux before it): residual gathered from fs (or read from
+// Rin, SoA 5 x nElem), local dt and the update in place. integ 0: RK, W_k = W0 - alpha*dt*R/V
+// (firstStage also stores W0); integ 1: explicit Euler, W0 is the persistent conservative state;
+// integ 2: no update. When rmsPartial != nullptr each block writes its partial sum of R_rho^2.
+void launchStage(const DevMesh &m, DevState s, const DevFree &q, const double *fs, const double *Rin, double *W0,
+                 double alpha, int firstStage, int integ, double *rmsPartial, double *Rout, hipStream_t st);
+// residual smoothing R0 -> R2 (R1 scratch, blockMax one double per 256 cells, flag[2] device ints:
+// flag[1] is sticky NaN and must start at 0)
+void launchSmooth(const double *R0, double *R1, double *R2, const int *nbr, const uint8_t *nface, int stride, int nElem,
+                  double *blockMax, int *flag, hipStream_t st);
 int stageBlocks(int nElem);
 // launch choices of the fused stage (they change speed only, never results)
 struct StageTuning {
@@ -99,8 +104,10 @@ struct StageTuning {
 // Fused RK stage: boundary ghosts, face fluxes (once per block, in LDS), residual gather, local
 // dt and update, reading `in` and writing `out` (interior and ghosts). Before the first update
 // after a state upload (machDerived == 0) the interior Mach number comes from the state's M slot.
+// integ 0: RK stage (W0 = the iteration's start state, written in the first stage); integ 1:
+// explicit Euler step (W0 = the persistent conservative state rho, rho u, rho v, rho w, rho E).
 void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const DevFree &q, int scheme,
-                      double *W0, double alpha, int firstStage, int machDerived, double *rmsPartial,
+                      int integ, double *W0, double alpha, int firstStage, int machDerived, double *rmsPartial,
                       const StageTuning &t, hipStream_t st);
 // deterministic sum of n doubles into out[0] (single workgroup, fixed order)
 void launchSum(const double *in, int n, double *out, hipStream_t st);

@@ -381,9 +381,30 @@ __device__ __forceinline__ double blockSum(double x, double *lds) {
 template<bool NT = false>
 __device__ __forceinline__ void updateCell(const DevMesh &m, const DevState &out, const DevFree &q, double *W0,
                                            double alpha, int firstStage, int e, const double r[5], double rho, double u,
-                                           double v, double w, double p, double E) {
+                                           double v, double w, double p, double E, int integ = 0) {
 	size_t n = (size_t) m.nElem;
 	double w0[5];
+	if (integ == 1) {
+		// explicit Euler (TimeIntegration.cpp:78-91, FlowField.cpp:103-126), W0 = persistent W
+		double dt = localDt<NT>(m, q, rho, u, v, w, p, e);
+		double coeff = -dt / ldStream<NT>(m.c.vol + e);
+		double nr = rho + r[0] * coeff;
+		double nru = W0[n + e] + r[1] * coeff;
+		double nrv = W0[2 * n + e] + r[2] * coeff;
+		double nrw = W0[3 * n + e] + r[3] * coeff;
+		double nrE = W0[4 * n + e] + r[4] * coeff;
+		W0[e] = nr, W0[n + e] = nru, W0[2 * n + e] = nrv, W0[3 * n + e] = nrw, W0[4 * n + e] = nrE;
+		double nu = nru / nr, nv = nrv / nr, nw = nrw / nr, nE = nrE / nr;
+		double g = q.gamma;
+		double np = (g - 1) * nr * (nE - (nu * nu + nv * nv + nw * nw) * 0.5);
+		double nH = nE + (np / nr);
+		double2 *c = reinterpret_cast<double2 *>(cellPtr(out, e));
+		c[0] = make_double2(nr, nu);
+		c[1] = make_double2(nv, nw);
+		c[2] = make_double2(np, nE);
+		c[3].x = nH;
+		return;
+	}
 	if (firstStage) {
 		w0[0] = rho;
 		w0[1] = u * rho;
@@ -414,26 +435,129 @@ __device__ __forceinline__ void updateCell(const DevMesh &m, const DevState &out
 	c[3].x = nH;
 }
 
-__global__ void __launch_bounds__(kBlock) kStage(DevMesh m, DevState s, DevFree q, const double *fs, double *W0, double alpha,
-                                                 int firstStage, double *rmsPartial, double *Rout) {
+// integ 0: RK stage, 1: explicit Euler step, 2: no update (TIME_INTEGRATION= 1 only recomputes
+// dt, EulerSolver.cpp:474-480). The residual is gathered from fs, or read from Rin when given.
+__global__ void __launch_bounds__(kBlock) kStage(DevMesh m, DevState s, DevFree q, const double *fs, const double *Rin,
+                                                 double *W0, double alpha, int firstStage, int integ,
+                                                 double *rmsPartial, double *Rout) {
 	__shared__ double lds[kBlock];
 	int e = blockIdx.x * kBlock + threadIdx.x;
 	double r2 = 0.0;
 	if (e < m.nElem) {
 		size_t n = (size_t) m.nElem;
 		double r[5];
-		gatherCell(m, fs, e, r);
+		if (Rin) {
+#pragma unroll
+			for (int k = 0; k < 5; k++) r[k] = Rin[k * n + e];
+		} else {
+			gatherCell(m, fs, e, r);
+		}
 		if (Rout) {
 #pragma unroll
 			for (int k = 0; k < 5; k++) Rout[k * n + e] = r[k];
 		}
-		const double *c = cellPtr(s, e);
-		updateCell(m, s, q, W0, alpha, firstStage, e, r, c[S_RHO], c[S_U], c[S_V], c[S_W], c[S_P], c[S_E]);
+		if (integ != 2) {
+			const double *c = cellPtr(s, e);
+			updateCell(m, s, q, W0, alpha, firstStage, e, r, c[S_RHO], c[S_U], c[S_V], c[S_W], c[S_P], c[S_E], integ);
+		}
 		r2 = r[0] * r[0];
 	}
 	if (rmsPartial) {
 		double t = blockSum(r2, lds);
 		if (threadIdx.x == 0) rmsPartial[blockIdx.x] = t;
+	}
+}
+
+// EulerSolver::smoothResiduals (EulerSolver.cpp:483-518) as it runs: one Jacobi sweep
+// R1_e = (R0_e + sum over interior neighbours s of R0_s * 0.25) / (1 + 0.25 * faces), neighbours in
+// local-face order; a second sweep from R1 follows when the first changed any value by more than
+// 1e-16 (the loop's own test scans an emptied vector from its second pass on, so it ends there).
+// nbr: per cell (device ids) the neighbour of each local face, -2 for a ghost (counted, not summed),
+// -1 padding; nface: faces per cell.
+__device__ __forceinline__ void smoothCell(const double *R0, const double *last, const int *nbr, const uint8_t *nface,
+                                           int stride, size_t n, int e, double out[5]) {
+	double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+	const int *nb = nbr + (size_t) e * stride;
+	for (int i = 0; i < stride; i++) {
+		const int s = nb[i];
+		if (s == -1) break;
+		if (s >= 0) {
+#pragma unroll
+			for (int k = 0; k < 5; k++) sum[k] += last[k * n + s] * 0.25;
+		}
+	}
+	const double den = 1 + nface[e] * 0.25;
+#pragma unroll
+	for (int k = 0; k < 5; k++) out[k] = (R0[k * n + e] + sum[k]) / den;
+}
+
+// first sweep: R1 and the largest change per block (NaN when a change is NaN)
+__global__ void __launch_bounds__(kBlock) kSmooth1(const double *R0, double *R1, const int *nbr, const uint8_t *nface,
+                                                   int stride, int nElem, double *blockMax) {
+	__shared__ double lds[kBlock];
+	const int e = blockIdx.x * kBlock + threadIdx.x;
+	const size_t n = (size_t) nElem;
+	double d = 0.0;
+	if (e < nElem) {
+		double o[5];
+		smoothCell(R0, R0, nbr, nface, stride, n, e, o);
+#pragma unroll
+		for (int k = 0; k < 5; k++) {
+			R1[k * n + e] = o[k];
+			const double c = fabs(R0[k * n + e] - o[k]);
+			d = (c > d || c != c) ? c : d;
+		}
+	}
+	lds[threadIdx.x] = d;
+	__syncthreads();
+	for (int off = kBlock / 2; off > 0; off >>= 1) {
+		if ((int) threadIdx.x < off) {
+			const double a = lds[threadIdx.x], b = lds[threadIdx.x + off];
+			lds[threadIdx.x] = (b > a || b != b) ? b : a;
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) blockMax[blockIdx.x] = lds[0];
+}
+
+// flag[0] = 1 when the first sweep changed something by more than 1e-16, flag[1] = 1 on a NaN change
+__global__ void __launch_bounds__(kBlock) kSmoothDecide(const double *blockMax, int nb, int *flag) {
+	__shared__ double lds[kBlock];
+	double d = 0.0;
+	for (int i = threadIdx.x; i < nb; i += kBlock) {
+		const double c = blockMax[i];
+		d = (c > d || c != c) ? c : d;
+	}
+	lds[threadIdx.x] = d;
+	__syncthreads();
+	for (int off = kBlock / 2; off > 0; off >>= 1) {
+		if ((int) threadIdx.x < off) {
+			const double a = lds[threadIdx.x], b = lds[threadIdx.x + off];
+			lds[threadIdx.x] = (b > a || b != b) ? b : a;
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		const double err = lds[0];
+		flag[0] = err > 1e-16 ? 1 : 0;
+		flag[1] = flag[1] | (err != err ? 1 : 0);
+	}
+}
+
+// second sweep from R1 into R2 when flag[0], else R2 = R1
+__global__ void __launch_bounds__(kBlock) kSmooth2(const double *R0, const double *R1, double *R2, const int *nbr,
+                                                   const uint8_t *nface, int stride, int nElem, const int *flag) {
+	const int e = blockIdx.x * kBlock + threadIdx.x;
+	if (e >= nElem) return;
+	const size_t n = (size_t) nElem;
+	if (flag[0]) {
+		double o[5];
+		smoothCell(R0, R1, nbr, nface, stride, n, e, o);
+#pragma unroll
+		for (int k = 0; k < 5; k++) R2[k * n + e] = o[k];
+	} else {
+#pragma unroll
+		for (int k = 0; k < 5; k++) R2[k * n + e] = R1[k * n + e];
 	}
 }
 
@@ -452,7 +576,7 @@ __device__ __forceinline__ int logicalBlock(int w, int nBlocks, int xcds) {
 // and its halo (the outside cells its faces read) are staged in LDS as rho, u, v, w, p, H; the
 // update's inputs (W0, V, sum|S|, face slots) are loaded into registers. The flux loop then reads
 // LDS only, except for boundary faces, whose ghost cells live in global memory.
-template<int SCHEME, bool NT, bool MK>
+template<int SCHEME, bool NT, bool MK, int INTEG>
 __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out, DevFree q,
                                                       double *W0, double alpha, int firstStage, int machDerived,
                                                       double *rmsPartial, int xcds) {
@@ -480,7 +604,7 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 	double w0[5], vol = 0, sx = 0, sy = 0, sz = 0;
 	uint16_t code[6];
 	if (own) {
-		if (!firstStage) {
+		if (INTEG == 1 || !firstStage) {
 #pragma unroll
 			for (int k = 0; k < 5; k++) w0[k] = ldStream<NT>(W0 + k * n + e);
 		}
@@ -596,7 +720,7 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 			}
 		}
 		const double rho = ru.x, u = ru.y, v = vw.x, w = vw.y, pp = pE.x;
-		if (firstStage) {
+		if (INTEG == 0 && firstStage) {
 			// W0 = (rho, u*rho, v*rho, w*rho, E*rho), EulerSolver.cpp:431-438
 			w0[0] = rho;
 			w0[1] = u * rho;
@@ -613,12 +737,29 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 		double ly = 0.5 * (fabs(v) + c) * sy;
 		double lz = 0.5 * (fabs(w) + c) * sz;
 		double dt = q.cfl * vol / (lx + ly + lz);
-		const double yv = MK ? 1.0 / vol : 0.0;
-		double nr = w0[0] - quot<MK>(alpha * dt * r[0], vol, yv);
-		double nru = w0[1] - quot<MK>(alpha * dt * r[1], vol, yv);
-		double nrv = w0[2] - quot<MK>(alpha * dt * r[2], vol, yv);
-		double nrw = w0[3] - quot<MK>(alpha * dt * r[3], vol, yv);
-		double nrE = w0[4] - quot<MK>(alpha * dt * r[4], vol, yv);
+		double nr, nru, nrv, nrw, nrE;
+		if (INTEG == 1) {
+			// explicit Euler: dW = R * (-dt / V) (TimeIntegration.cpp:78-91), FlowField::Update
+			// (FlowField.cpp:103-126): rho += dW.rho, (rho u, rho v, rho w, rho E) += dW
+			const double coeff = -dt / vol;
+			nr = rho + r[0] * coeff;
+			nru = w0[1] + r[1] * coeff;
+			nrv = w0[2] + r[2] * coeff;
+			nrw = w0[3] + r[3] * coeff;
+			nrE = w0[4] + r[4] * coeff;
+			W0[e] = nr;
+			W0[n + e] = nru;
+			W0[2 * n + e] = nrv;
+			W0[3 * n + e] = nrw;
+			W0[4 * n + e] = nrE;
+		} else {
+			const double yv = MK ? 1.0 / vol : 0.0;
+			nr = w0[0] - quot<MK>(alpha * dt * r[0], vol, yv);
+			nru = w0[1] - quot<MK>(alpha * dt * r[1], vol, yv);
+			nrv = w0[2] - quot<MK>(alpha * dt * r[2], vol, yv);
+			nrw = w0[3] - quot<MK>(alpha * dt * r[3], vol, yv);
+			nrE = w0[4] - quot<MK>(alpha * dt * r[4], vol, yv);
+		}
 		const double yr = MK ? 1.0 / nr : 0.0;
 		double nu = quot<MK>(nru, nr, yr), nv = quot<MK>(nrv, nr, yr), nw = quot<MK>(nrw, nr, yr);
 		double nE = quot<MK>(nrE, nr, yr);
@@ -717,23 +858,38 @@ void launchTimestep(const DevMesh &m, DevState s, const DevFree &q, double *dt, 
 	hipLaunchKernelGGL(kTimestep, dim3(blocksFor(m.nElem)), dim3(kBlock), 0, st, m, s, q, dt);
 }
 
-void launchStage(const DevMesh &m, DevState s, const DevFree &q, const double *fs, double *W0, double alpha,
-                 int firstStage, double *rmsPartial, double *Rout, hipStream_t st) {
+void launchStage(const DevMesh &m, DevState s, const DevFree &q, const double *fs, const double *Rin, double *W0,
+                 double alpha, int firstStage, int integ, double *rmsPartial, double *Rout, hipStream_t st) {
 	if (m.nElem == 0) return;
-	hipLaunchKernelGGL(kStage, dim3(blocksFor(m.nElem)), dim3(kBlock), 0, st, m, s, q, fs, W0, alpha, firstStage,
-	                   rmsPartial, Rout);
+	hipLaunchKernelGGL(kStage, dim3(blocksFor(m.nElem)), dim3(kBlock), 0, st, m, s, q, fs, Rin, W0, alpha, firstStage,
+	                   integ, rmsPartial, Rout);
+}
+
+void launchSmooth(const double *R0, double *R1, double *R2, const int *nbr, const uint8_t *nface, int stride, int nElem,
+                  double *blockMax, int *flag, hipStream_t st) {
+	if (nElem == 0) return;
+	const int nb = blocksFor(nElem);
+	hipLaunchKernelGGL(kSmooth1, dim3(nb), dim3(kBlock), 0, st, R0, R1, nbr, nface, stride, nElem, blockMax);
+	hipLaunchKernelGGL(kSmoothDecide, dim3(1), dim3(kBlock), 0, st, blockMax, nb, flag);
+	hipLaunchKernelGGL(kSmooth2, dim3(nb), dim3(kBlock), 0, st, R0, R1, R2, nbr, nface, stride, nElem, flag);
 }
 
 void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const DevFree &q, int scheme,
-                      double *W0, double alpha, int firstStage, int machDerived, double *rmsPartial,
+                      int integ, double *W0, double alpha, int firstStage, int machDerived, double *rmsPartial,
                       const StageTuning &t, hipStream_t st) {
 	if (p.nBlocks == 0) return;
 	size_t words = (size_t) 6 * (kBlock + p.maxHalo) + (size_t) 5 * p.maxSlots;
 	size_t lds = sizeof(double) * (words > (size_t) kBlock ? words : (size_t) kBlock);
 	const int x = t.xcds;
-#define E3D_LAUNCH(S, N, K)                                                                                           \
-	hipLaunchKernelGGL((kStageFused<S, N, K>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, q, W0, alpha, \
-	                   firstStage, machDerived, rmsPartial, x)
+#define E3D_LAUNCH(S, N, K)                                                                                          \
+	do {                                                                                                             \
+		if (integ == 1)                                                                                              \
+			hipLaunchKernelGGL((kStageFused<S, N, K, 1>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, q, \
+			                   W0, alpha, 0, machDerived, rmsPartial, x);                                          \
+		else                                                                                                         \
+			hipLaunchKernelGGL((kStageFused<S, N, K, 0>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, q, \
+			                   W0, alpha, firstStage, machDerived, rmsPartial, x);                                 \
+	} while (0)
 #define E3D_LAUNCH_K(S, N)               \
 	if (t.sharedRecip) E3D_LAUNCH(S, N, true); \
 	else E3D_LAUNCH(S, N, false)

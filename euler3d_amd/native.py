@@ -14,7 +14,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libe3d_hip.so"
 
 E3D_OK = 0
-ERRORS = {1: "argument", 2: "io", 3: "mesh", 4: "device", 5: "unsupported", 6: "busy"}
+ERRORS = {1: "argument", 2: "io", 3: "mesh", 4: "device", 5: "unsupported", 6: "busy", 7: "numeric"}
 E3D_ERR_BUSY = 6
 
 

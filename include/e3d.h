@@ -7,6 +7,8 @@
  *   EulerSolver(FlowField&, MPIHandler&, Mesh<MeshPartition>&, SimConfig&, Metrics&)
  *       include/solver/EulerSolver.hpp:19-23          -> e3d_create
  *   EulerSolver::Run()            src/solver/EulerSolver.cpp:60-167   -> e3d_run / e3d_iterate
+ *     (TIME_INTEGRATION 0/1/2: EulerExplicit / EulerImplicit / RungeKutta, EulerSolver.cpp:419-472;
+ *      RESIDUAL_SMOOTHING: smoothResiduals, EulerSolver.cpp:483-518)
  *   EulerSolver::updateBC()       src/solver/EulerSolver.cpp:201-251  -> e3d_update_bc
  *   Solver::Roe / Solver::Fc      src/solver/ConvectiveFluxesSchemes.cpp:10-187 -> e3d_face_flux
  *   EulerSolver::computeResidual  src/solver/EulerSolver.cpp:303-359  -> e3d_residual
@@ -43,6 +45,7 @@ extern "C" {
 #define E3D_ERR_DEVICE 4
 #define E3D_ERR_UNSUPPORTED 5
 #define E3D_ERR_BUSY 6 /* batch: no free case slot or the HBM budget is exhausted; retry after a removal */
+#define E3D_ERR_NUMERIC 7 /* a NaN residual in the residual smoothing (the reference exits the process) */
 
 #define E3D_SCHEME_ROE 0
 #define E3D_SCHEME_AUSM 1
@@ -114,7 +117,8 @@ void e3d_mesh_free(e3d_mesh *m);
 /* Uploads a preprocessed mesh to HIP device `device` and initialises the freestream state
  * (FlowField::Initialize, src/solver/FlowField.cpp:68-101). */
 int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out);
-/* fields = {rho, u, v, w, p, E, H, M}, each n_elem + n_ghost values */
+/* fields = {rho, u, v, w, p, E, H, M}, each n_elem + n_ghost values. The explicit Euler step's
+ * conservative state (FlowField's rho u, rho v, rho w, rho E) is set to u*rho, v*rho, w*rho, E*rho. */
 int e3d_set_state(e3d_ctx *c, const double *const fields[8]);
 int e3d_get_state(e3d_ctx *c, double *const fields[8]);
 int e3d_update_bc(e3d_ctx *c);

@@ -384,6 +384,193 @@ void or_aero(const or_mesh *m, const or_params *q, const or_state *s, double cf[
 	}
 }
 
+/* EulerSolver::smoothResiduals, EulerSolver.cpp:483-518: Jacobi sweeps
+ * R_e = (R0_e + sum over interior neighbours of 0.25 R_s) / (1 + 0.25 * (number of faces)), the
+ * neighbours taken in local-face order. The loop is written as "repeat until max |change| <=
+ * 1e-16", but the vector of changes is cleared at the end of each pass (:515) and the next pass
+ * writes its changes through begin() of the empty vector and scans zero elements, so the second
+ * pass always reports no change: one sweep when the first changes nothing, else exactly two.
+ * Only the first pass checks for NaN (ResidualVar::findMax exits the process on one). */
+int or_smooth(const or_mesh *m, const or_params *q, double *R) {
+	const double epsilon = 0.25;
+	const size_t n = (size_t) m->nElem;
+	double *orig = (double *) malloc(sizeof(double) * 5 * n);
+	double *last = (double *) malloc(sizeof(double) * 5 * n);
+	memcpy(orig, R, sizeof(double) * 5 * n);
+	int sweeps = 0;
+	double error;
+	do {
+		error = 0.0;
+		memcpy(last, R, sizeof(double) * 5 * n);
+		for (size_t e = 0; e < n; e++) {
+			double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+			int b = m->element2faceStart[e], en = m->element2faceStart[e + 1];
+			int ns = en - b;
+			for (int i = b; i < en; i++) {
+				int s = q->element2element[i];
+				if (s < m->nElem)
+					for (int k = 0; k < 5; k++) sum[k] += last[5 * (size_t) s + k] * epsilon;
+			}
+			double den = 1 + ns * epsilon;
+			for (int k = 0; k < 5; k++) R[5 * e + k] = (orig[5 * e + k] + sum[k]) / den;
+		}
+		if (sweeps == 0) {
+			for (size_t i = 0; i < 5 * n; i++) {
+				double d = fabs(last[i] - R[i]);
+				if (isnan(d)) {
+					free(orig);
+					free(last);
+					return -1;
+				}
+				if (d > error) error = d;
+			}
+		}
+		sweeps++;
+	} while (error > 1e-16);
+	free(orig);
+	free(last);
+	return sweeps;
+}
+
+/* FlowField::Initialize, FlowField.cpp:83-95: the conservative arrays start at rho_inf * (u, v, w, E)_inf */
+void or_init_conservative(const or_mesh *m, const or_params *q, const or_state *s, double *W) {
+	(void) s;
+	for (int e = 0; e < m->nElem; e++) {
+		W[5 * (size_t) e] = q->rho_inf;
+		W[5 * (size_t) e + 1] = q->rho_inf * q->u_inf;
+		W[5 * (size_t) e + 2] = q->rho_inf * q->v_inf;
+		W[5 * (size_t) e + 3] = q->rho_inf * q->w_inf;
+		W[5 * (size_t) e + 4] = q->rho_inf * q->E_inf;
+	}
+}
+
+/* EulerSolver::EulerExplicit (EulerSolver.cpp:419-423): dt, dW = R * (-dt / V) (ExplicitEuler,
+ * TimeIntegration.cpp:78-91), FlowField::Update (FlowField.cpp:103-126): rho += dW.rho (the
+ * density array is both primitive and conservative), rho u += dW.rhoU, ..., then the primitives */
+void or_euler(const or_mesh *m, const or_params *q, or_state *s, double *W, const double *R) {
+	double g = q->gamma;
+	size_t n = (size_t) m->nElem;
+	double *dt = (double *) malloc(sizeof(double) * n);
+	or_timestep(m, q, s, dt);
+	for (size_t e = 0; e < n; e++) {
+		double coeff = -dt[e] / m->cellVolume[e];
+		double d[5];
+		for (int k = 0; k < 5; k++) d[k] = R[5 * e + k] * coeff;
+		double rho = s->rho[e] + d[0];
+		W[5 * e] = rho;
+		W[5 * e + 1] += d[1];
+		W[5 * e + 2] += d[2];
+		W[5 * e + 3] += d[3];
+		W[5 * e + 4] += d[4];
+		double u = W[5 * e + 1] / rho, v = W[5 * e + 2] / rho, w = W[5 * e + 3] / rho, E = W[5 * e + 4] / rho;
+		double p = (g - 1) * rho * (E - (u * u + v * v + w * w) / 2.0);
+		double H = E + (p / rho);
+		double M = sqrt(g * (p / rho)) / sqrt((u * u + v * v + w * w));
+		s->rho[e] = rho;
+		s->u[e] = u;
+		s->v[e] = v;
+		s->w[e] = w;
+		s->E[e] = E;
+		s->p[e] = p;
+		s->H[e] = H;
+		s->M[e] = M;
+	}
+	free(dt);
+}
+
+/* EulerSolver::RungeKutta with RESIDUAL_SMOOTHING (EulerSolver.cpp:450-456: smoothing after each
+ * stage's residual) */
+static int rk5_smooth(const or_mesh *m, const or_params *q, or_state *s, double *R) {
+	static const double alphas[4] = {0.1263, 0.2375, 0.4414, 1.0};
+	size_t n = (size_t) m->nElem;
+	double *W0 = (double *) malloc(sizeof(double) * 5 * n);
+	double *dt = (double *) malloc(sizeof(double) * n);
+	for (size_t i = 0; i < n; i++) {
+		double rho = s->rho[i];
+		W0[5 * i] = rho;
+		W0[5 * i + 1] = s->u[i] * rho;
+		W0[5 * i + 2] = s->v[i] * rho;
+		W0[5 * i + 3] = s->w[i] * rho;
+		W0[5 * i + 4] = s->E[i] * rho;
+	}
+	or_timestep(m, q, s, dt);
+	rk_update(m, q, s, W0, R, dt, 0.0533);
+	int rc = 0;
+	for (int a = 0; a < 4 && rc >= 0; a++) {
+		or_update_bc(m, q, s);
+		or_residual(m, q, s, R);
+		if (q->residual_smoothing) rc = or_smooth(m, q, R);
+		or_timestep(m, q, s, dt);
+		rk_update(m, q, s, W0, R, dt, alphas[a]);
+	}
+	free(W0);
+	free(dt);
+	return rc < 0 ? -1 : 0;
+}
+
+/* EulerSolver::Run, EulerSolver.cpp:60-167, any TIME_INTEGRATION and RESIDUAL_SMOOTHING */
+int or_run_any(const or_mesh *m, const or_params *q, or_state *s, double *W, double *R, double *rows, int max_rows,
+               int *iters, double coeffs[3], double cfOut[3], double cmOut[3]) {
+	int nb = 0, counter = 0, nrows = 0;
+	double CL = 0, CD = 0, CM = 0;
+	double cf[3] = {0, 0, 0}, cm[3] = {0, 0, 0};
+#define E3D_ROW(it, rms)                                                          \
+	do {                                                                          \
+		CL = cf[q->orient[0][0]] * q->orient[0][1];                               \
+		CD = cf[q->orient[1][0]] * q->orient[1][1];                               \
+		CM = cm[q->orient[2][0]] * q->orient[2][1];                               \
+		if (nrows < max_rows) {                                                   \
+			double *r_ = &rows[8 * (size_t) nrows];                               \
+			r_[0] = (it);                                                         \
+			r_[1] = (rms);                                                        \
+			r_[2] = cf[0], r_[3] = cf[1], r_[4] = cf[2];                          \
+			r_[5] = cm[0], r_[6] = cm[1], r_[7] = cm[2];                          \
+		}                                                                         \
+		nrows++;                                                                  \
+	} while (0)
+	while (nb < q->max_iter) {
+		or_update_bc(m, q, s);
+		or_residual(m, q, s, R);
+		if (q->residual_smoothing && or_smooth(m, q, R) < 0) return -1;
+		if (q->time_integration == 2) {
+			if (q->residual_smoothing) {
+				if (rk5_smooth(m, q, s, R) < 0) return -1;
+			} else {
+				or_rk5(m, q, s, R);
+			}
+		} else if (q->time_integration == 0) {
+			or_euler(m, q, s, W, R);
+		}
+		/* TIME_INTEGRATION 1 (EulerImplicit, EulerSolver.cpp:474-480) only recomputes dt */
+		double err = or_rms_sum(R, m->nElem);
+		double rms = sqrt(err / m->nElem);
+		if (rms < q->min_residual) {
+			E3D_ROW(nb, rms);
+			break;
+		}
+		nb += 1;
+		if (nb % q->sampling == 0) {
+			double oCL = CL, oCD = CD, oCM = CM;
+			or_aero(m, q, s, cf, cm);
+			E3D_ROW(nb, rms);
+			double tol = q->min_aerocoeff_error;
+			int crit = fabs(oCL - CL) < tol && fabs(oCD - CD) < tol && fabs(oCM - CM) < tol;
+			counter = crit ? counter + 1 : 0;
+			if (counter >= 2) break;
+		}
+	}
+#undef E3D_ROW
+	*iters = nb;
+	coeffs[0] = CL;
+	coeffs[1] = CD;
+	coeffs[2] = CM;
+	for (int k = 0; k < 3; k++) {
+		cfOut[k] = cf[k];
+		cmOut[k] = cm[k];
+	}
+	return nrows;
+}
+
 /* EulerSolver::Run, EulerSolver.cpp:60-167, TIME_INTEGRATION = 2 (RK5), no residual smoothing */
 int or_run(const or_mesh *m, const or_params *q, or_state *s, double *R, double *rows, int max_rows, int *iters,
            double coeffs[3], double cfOut[3], double cmOut[3]) {

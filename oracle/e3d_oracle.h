@@ -18,6 +18,10 @@
  *   or_rms_sum         EulerSolver.cpp:382-389     golden fixtures (rk_rms_sum, run_rms_sum)
  *   or_aero            AeroCoefficients.cpp:5-52   golden fixtures (force/moment coeff, CL/CD/CM)
  *   or_run             EulerSolver.cpp:60-167      golden fixtures (residuals.dat, final state)
+ *   or_smooth          EulerSolver.cpp:483-518     golden fixtures (runs with RESIDUAL_SMOOTHING= 1)
+ *   or_euler           EulerSolver.cpp:419-423, TimeIntegration.cpp:78-91, FlowField.cpp:103-126
+ *                                                  golden fixtures (runs with TIME_INTEGRATION= 0)
+ *   or_run_any         EulerSolver.cpp:60-167      golden fixtures (TIME_INTEGRATION 0/1/2, smoothing)
  *   or_ausm            no reference implementation exists (ConvectiveFluxesSchemes.h:27-30 only
  *                      declares it): PARITY UNPINNED. Restates Liou & Steffen, J. Comput. Phys.
  *                      107 (1993) 23-39, and is checked by invariants only (consistency
@@ -55,6 +59,9 @@ typedef struct {
 	double min_residual, min_aerocoeff_error;
 	/* filled by or_freestream */
 	double rho_inf, p_inf, u_inf, v_inf, w_inf, E_inf, H_inf, M_inf;
+	int time_integration;   /* TIME_INTEGRATION: 0 explicit Euler, 1 implicit Euler (no update), 2 RK5 */
+	int residual_smoothing; /* RESIDUAL_SMOOTHING */
+	const int *element2element; /* neighbour per local face (CSR by element2faceStart), for smoothing */
 } or_params;
 
 typedef struct {
@@ -82,6 +89,21 @@ void or_aero(const or_mesh *m, const or_params *prm, const or_state *s, double c
  * CMx, CMy, CMz as residuals.dat holds them) and returns the number of rows; *iters receives the
  * final iteration count and coeffs[3] CL, CD, CM. R (5*nElem) is scratch and holds the last
  * residual on exit. */
+/* Implicit residual smoothing as the reference runs it: one Jacobi sweep when the first changes
+ * nothing, otherwise exactly two (its convergence test scans an emptied vector from the second
+ * pass on). Returns the number of sweeps, or -1 when the first sweep meets a NaN residual (the
+ * reference exits the process). */
+int or_smooth(const or_mesh *m, const or_params *prm, double *R);
+/* One explicit Euler step: W (5*nElem conservative, kept between steps) += R * (-dt / V), then the
+ * primitives from W. */
+void or_euler(const or_mesh *m, const or_params *prm, or_state *s, double *W, const double *R);
+/* Run loop for every TIME_INTEGRATION / RESIDUAL_SMOOTHING choice. W (5*nElem) is the explicit
+ * Euler conservative state (FlowField's rho u, rho v, rho w, rho E arrays; set to the freestream
+ * by or_init_conservative). Returns the number of rows as or_run does, or -1 on a NaN residual in
+ * the smoothing. */
+void or_init_conservative(const or_mesh *m, const or_params *prm, const or_state *s, double *W);
+int or_run_any(const or_mesh *m, const or_params *prm, or_state *s, double *W, double *R, double *rows,
+               int max_rows, int *iters, double coeffs[3], double cf[3], double cm[3]);
 int or_run(const or_mesh *m, const or_params *prm, or_state *s, double *R, double *rows, int max_rows,
            int *iters, double coeffs[3], double cf[3], double cm[3]);
 

@@ -34,7 +34,8 @@ class OrParams(C.Structure):
                 ("orient", (C.c_int * 2) * 3), ("scheme", C.c_int), ("max_iter", C.c_int), ("sampling", C.c_int),
                 ("min_residual", C.c_double), ("min_aerocoeff_error", C.c_double),
                 ("rho_inf", C.c_double), ("p_inf", C.c_double), ("u_inf", C.c_double), ("v_inf", C.c_double),
-                ("w_inf", C.c_double), ("E_inf", C.c_double), ("H_inf", C.c_double), ("M_inf", C.c_double)]
+                ("w_inf", C.c_double), ("E_inf", C.c_double), ("H_inf", C.c_double), ("M_inf", C.c_double),
+                ("time_integration", C.c_int), ("residual_smoothing", C.c_int), ("element2element", IP)]
 
 
 class OrState(C.Structure):
@@ -64,6 +65,10 @@ def lib():
             "or_rms_sum": (C.c_double, [DP, C.c_int]),
             "or_aero": (None, [m, p, s, DP, DP]),
             "or_run": (C.c_int, [m, p, s, DP, DP, C.c_int, IP, DP, DP, DP]),
+            "or_smooth": (C.c_int, [m, p, DP]),
+            "or_euler": (None, [m, p, s, DP, DP]),
+            "or_init_conservative": (None, [m, p, s, DP]),
+            "or_run_any": (C.c_int, [m, p, s, DP, DP, DP, C.c_int, IP, DP, DP, DP]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -130,6 +135,14 @@ class Oracle:
         q.sampling = params.get("sampling", 1)
         q.min_residual = params.get("min_residual", 0.0)
         q.min_aerocoeff_error = params.get("min_aerocoeff_error", 0.0)
+        q.time_integration = params.get("time_integration", 2)
+        q.residual_smoothing = params.get("residual_smoothing", 0)
+        if "element2element" in arrays:
+            a, ptr = _i(arrays["element2element"])
+            self._keep.append(a)
+            q.element2element = ptr
+        elif q.residual_smoothing:
+            raise ValueError("residual smoothing needs the element2element array")
         lib().or_freestream(C.byref(q))
         self.params = q
         self.n_elem, self.n_face, self.n_ghost = m.nElem, m.nFace, m.nGhost
@@ -138,6 +151,9 @@ class Oracle:
         self._st = OrState(*[self.state[k].ctypes.data_as(DP) for k in FIELDS])
         lib().or_init_state(C.byref(m), C.byref(q), C.byref(self._st))
         self.R = np.zeros(5 * self.n_elem)
+        # explicit Euler's conservative state (FlowField's rho u, rho v, rho w, rho E arrays)
+        self.W = np.empty(5 * self.n_elem)
+        lib().or_init_conservative(C.byref(m), C.byref(q), C.byref(self._st), self.W.ctypes.data_as(DP))
 
     def set_state(self, st: dict) -> None:
         for k in FIELDS:
@@ -177,6 +193,32 @@ class Oracle:
             out[i] = lib().or_rms_sum(self.R.ctypes.data_as(DP), self.n_elem)
         return out
 
+    def smooth(self) -> int:
+        """smoothResiduals on self.R; the number of sweeps (-1: a NaN residual)."""
+        return lib().or_smooth(C.byref(self.mesh), C.byref(self.params), self.R.ctypes.data_as(DP))
+
+    def euler(self):
+        """EulerExplicit from the residual in self.R."""
+        lib().or_euler(C.byref(self.mesh), C.byref(self.params), C.byref(self._st), self.W.ctypes.data_as(DP),
+                       self.R.ctypes.data_as(DP))
+
+    def iterate_any(self, n: int) -> np.ndarray:
+        """n iterations of Run's body with the configured integrator and smoothing."""
+        out = np.empty(n)
+        for i in range(n):
+            self.update_bc()
+            self.residual()
+            if self.params.residual_smoothing and self.smooth() < 0:
+                raise FloatingPointError("NaN residual in the smoothing")
+            if self.params.time_integration == 2:
+                if self.params.residual_smoothing:
+                    raise NotImplementedError("use run() for RK5 with smoothing")
+                self.rk5()
+            elif self.params.time_integration == 0:
+                self.euler()
+            out[i] = lib().or_rms_sum(self.R.ctypes.data_as(DP), self.n_elem)
+        return out
+
     def rms_sum(self) -> float:
         return lib().or_rms_sum(self.R.ctypes.data_as(DP), self.n_elem)
 
@@ -192,9 +234,11 @@ class Oracle:
         rows = np.zeros(8 * n)
         iters = C.c_int()
         co, cf, cm = np.empty(3), np.empty(3), np.empty(3)
-        nrows = lib().or_run(C.byref(self.mesh), C.byref(self.params), C.byref(self._st), self.R.ctypes.data_as(DP),
-                             rows.ctypes.data_as(DP), n, C.byref(iters), co.ctypes.data_as(DP),
-                             cf.ctypes.data_as(DP), cm.ctypes.data_as(DP))
+        nrows = lib().or_run_any(C.byref(self.mesh), C.byref(self.params), C.byref(self._st),
+                                 self.W.ctypes.data_as(DP), self.R.ctypes.data_as(DP), rows.ctypes.data_as(DP), n,
+                                 C.byref(iters), co.ctypes.data_as(DP), cf.ctypes.data_as(DP), cm.ctypes.data_as(DP))
+        if nrows < 0:
+            raise FloatingPointError("NaN residual in the smoothing")
         k = min(nrows, n)
         return {"rows": rows[: 8 * k].reshape(k, 8), "iterations": iters.value, "CL": co[0], "CD": co[1],
                 "CM": co[2], "force_coeff": cf, "moment_coeff": cm}

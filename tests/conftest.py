@@ -16,6 +16,9 @@ sys.path.insert(0, str(ROOT / "oracle"))
 
 FIELDS = ("rho", "u", "v", "w", "p", "E", "H", "M")
 SMALL_CASES = ("block8", "naca9x9x5", "naca_small", "bump_small", "wing_small")
+# run-mode fixtures of the other integrators -> the fixture whose mesh they use
+INTEGRATOR_CASES = {"naca_euler": "naca_small", "naca_implicit": "naca_small", "bump_smooth": "bump_small",
+                    "wing_euler_smooth": "wing_small"}
 MESH_KEYS_I = ("face2element", "face2nodeStart", "face2node", "element2faceStart", "element2face",
                "element2element", "wallGhost", "wallCell", "wallFace", "farGhost", "farCell", "farFace",
                "symGhost", "symCell", "symFace")
@@ -49,6 +52,14 @@ def golden_params(g: dict) -> dict:
                 max_iter=int(p[11]), sampling=1, min_residual=0.0, min_aerocoeff_error=0.0, scheme=0)
 
 
+def integrator_params(g: dict) -> dict:
+    """golden_params plus TIME_INTEGRATION / RESIDUAL_SMOOTHING of the fixture (RK5, none if absent)."""
+    pd = golden_params(g)
+    ti, sm = (int(x) for x in g["integrator"]) if "integrator" in g else (2, 0)
+    pd.update(time_integration=ti, residual_smoothing=sm)
+    return pd
+
+
 def golden_seeds(g: dict) -> list[int]:
     return sorted({int(k[4:].split("_")[0]) for k in g if k.startswith("flux") and k.endswith("_in_rho")})
 
@@ -64,7 +75,8 @@ def native_params(pd: dict, **over):
     kw = dict(mach=pd["mach"], aoa_deg=pd["aoa_deg"], gamma=pd["gamma"], cfl=pd["cfl"], sref=pd["sref"],
               cref=pd["cref"], ref_point=pd["ref_point"], orient_cl=pd["orient"][0], orient_cd=pd["orient"][1],
               orient_cm=pd["orient"][2], max_iter=pd["max_iter"], sampling=pd["sampling"],
-              min_residual=pd["min_residual"], min_aerocoeff_error=pd["min_aerocoeff_error"], scheme=pd["scheme"])
+              min_residual=pd["min_residual"], min_aerocoeff_error=pd["min_aerocoeff_error"], scheme=pd["scheme"],
+              time_integration=pd.get("time_integration", 2), residual_smoothing=pd.get("residual_smoothing", 0))
     kw.update(over)
     return default_params(**kw)
 

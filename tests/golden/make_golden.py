@@ -16,6 +16,11 @@ Cases:
   wing_small  swept tapered wing, 25x13x9 nodes, M = 0.84, AoA = 3
   naca_full   the full 129x129x2 O-grid (16,384 hexes), 2000 iterations: residual history,
               coefficients and the final density and pressure (run mode only)
+  Run-mode cases of the other integrators on the meshes above (no new mesh files):
+  naca_euler          naca_small mesh, TIME_INTEGRATION= 0 (explicit Euler), 150 iterations
+  naca_implicit       naca_small mesh, TIME_INTEGRATION= 1 (implicit Euler: dt only), 5 iterations
+  bump_smooth         bump_small mesh, RK5 with RESIDUAL_SMOOTHING= 1, 10 iterations
+  wing_euler_smooth   wing_small mesh, explicit Euler with RESIDUAL_SMOOTHING= 1, 20 iterations
 
 Usage: python tests/golden/make_golden.py [case ...]
 """
@@ -62,8 +67,8 @@ GAMMA= 1.4
 GAS_CONSTANT= 287.058
 SPECIFIC_HEAT= 1004.7
 SCHEME= 0
-TIME_INTEGRATION= 2
-RESIDUAL_SMOOTHING= 0
+TIME_INTEGRATION= {ti}
+RESIDUAL_SMOOTHING= {smooth}
 CFL= {cfl}
 MIN_RESIDUAL= 0
 MIN_AEROCOEFF_ERROR= 0
@@ -96,6 +101,10 @@ CASES = {
                       mach=0.8, aoa=1.25, cfl=1.5, iters=2000, ocl=2, ocd=0, sref=0.1, cref=1.0,
                       ref=(0.25, 0.0, 0.0), seeds=(), run_only=True),
 }
+for _name, _base, _ti, _sm, _it in (("naca_euler", "naca_small", 0, 0, 150), ("naca_implicit", "naca_small", 1, 0, 5),
+                                     ("bump_smooth", "bump_small", 2, 1, 10),
+                                     ("wing_euler_smooth", "wing_small", 0, 1, 20)):
+    CASES[_name] = dict(CASES[_base], ti=_ti, smooth=_sm, iters=_it, seeds=(), run_mode=True, mesh_of=_base)
 
 
 def parse_dump(path: Path) -> dict:
@@ -170,12 +179,13 @@ def make_case(name: str, case: dict) -> None:
         cfg.write_text(CONFIG.format(mesh=par.name, par=str(par), out=str(td / "out"), ocl=case["ocl"],
                                      ocd=case["ocd"], rx=rx, ry=ry, rz=rz, sref=case["sref"], cref=case["cref"],
                                      sampling=1, mach=case["mach"], aoa=case["aoa"], cfl=case["cfl"],
-                                     iters=case["iters"]))
+                                     iters=case["iters"], ti=case.get("ti", 2), smooth=case.get("smooth", 0)))
         data = {}
         params = np.array([case["mach"], case["aoa"], 1.4, case["cfl"], case["sref"], case["cref"], rx, ry, rz,
                            case["ocl"], case["ocd"], case["iters"]], dtype=np.float64)
         data["params"] = params
-        if not case.get("run_only"):
+        data["integrator"] = np.array([case.get("ti", 2), case.get("smooth", 0)], dtype=np.int32)
+        if not case.get("run_only") and not case.get("run_mode"):
             mesh = run_harness("mesh", cfg, td / "mesh.txt")
             for k, v in mesh.items():
                 data["mesh_" + k] = v
@@ -198,7 +208,7 @@ def make_case(name: str, case: dict) -> None:
         data["run_summary_txt"] = np.frombuffer(summary.encode(), dtype=np.uint8)
         data["par_sha256"] = np.frombuffer(hashlib.sha256(text.encode()).hexdigest().encode(), dtype=np.uint8)
         np.savez_compressed(HERE / f"{name}.npz", **data)
-        if not case.get("run_only"):
+        if not case.get("run_only") and not case.get("run_mode"):
             # the full-size mesh is regenerated by the tests and checked against par_sha256
             with gzip.GzipFile(HERE / f"{name}.par.gz", "wb", mtime=0) as f:
                 f.write(text.encode())

@@ -11,8 +11,8 @@ import sys
 
 import numpy as np
 import pytest
-from conftest import FIELDS, SMALL_CASES, golden_mesh, golden_params, golden_seeds, load_golden, native_params, \
-    write_par
+from conftest import FIELDS, INTEGRATOR_CASES, SMALL_CASES, golden_mesh, golden_params, golden_seeds, \
+    integrator_params, load_golden, native_params, write_par
 from oracle_ctypes import Oracle
 
 from euler3d_amd import EulerSolver, Mesh
@@ -181,9 +181,35 @@ def test_gpu_error_paths(tmp_path):
     m = Mesh.box(2, 2, 2).preprocess(1.0)
     pd = golden_params(load_golden("block8"))
     with pytest.raises(E3DError, match="TIME_INTEGRATION"):
-        EulerSolver(m, native_params(pd, time_integration=0))
+        EulerSolver(m, native_params(pd, time_integration=3))
     with pytest.raises(E3DError, match="device"):
         EulerSolver(m, native_params(pd), device=99)
     with EulerSolver(m, native_params(pd)) as s:
         with pytest.raises(ValueError):
             s.set_state({k: np.zeros(3) for k in FIELDS})
+
+
+@pytest.mark.parametrize("path", ["fused", "unfused"])
+@pytest.mark.parametrize("case", sorted(INTEGRATOR_CASES))
+def test_gpu_other_integrators_match_reference(case, path, tmp_path, monkeypatch):
+    """Explicit Euler (TIME_INTEGRATION= 0), implicit Euler (1) and residual smoothing against full
+    reference runs: final state bit for bit, residual history, coefficients."""
+    say(f"[gpu] integrator case={case} path={path}")
+    if path == "unfused":
+        monkeypatch.setenv("E3D_STAGE_PATH", "unfused")
+    g = load_golden(case)
+    base = INTEGRATOR_CASES[case]
+    pd = integrator_params(g)
+    m = Mesh.read(str(write_par(base, tmp_path / f"{base}.par"))).preprocess(pd["cref"])
+    with EulerSolver(m, native_params(pd)) as s:
+        r = s.run()
+        st = s.get_state()
+    assert r["iterations"] == int(g["run_iterations"][0])
+    for k in FIELDS:
+        assert bits_equal(st[k], g["run_" + k]), (case, k)
+    rd = g["run_residuals_dat"]
+    assert r["rows"].shape == rd.shape
+    np.testing.assert_allclose(r["rows"], rd, rtol=6e-11, atol=1e-300)
+    for k in ("CL", "CD", "CM"):
+        ref = float(g["run_" + k][0])
+        assert abs(r[k] - ref) <= 1e-12 * max(abs(ref), 1e-300), (k, r[k], ref)

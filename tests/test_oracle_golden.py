@@ -9,7 +9,8 @@ from __future__ import annotations
 
 import numpy as np
 import pytest
-from conftest import FIELDS, SMALL_CASES, golden_mesh, golden_params, golden_seeds, load_golden
+from conftest import FIELDS, INTEGRATOR_CASES, SMALL_CASES, golden_mesh, golden_params, golden_seeds, \
+    integrator_params, load_golden
 from oracle_ctypes import Oracle
 
 
@@ -61,3 +62,50 @@ def test_oracle_run(case):
     assert r["rows"].shape == rd.shape
     # residuals.dat keeps 11 significant digits (%.10e, ResidualsFile.hpp:40)
     np.testing.assert_allclose(r["rows"], rd, rtol=6e-11, atol=1e-300)
+
+
+@pytest.mark.parametrize("case", sorted(INTEGRATOR_CASES))
+def test_oracle_run_other_integrators(case):
+    """Explicit Euler (TIME_INTEGRATION= 0), implicit Euler (1: dt only) and implicit residual
+    smoothing (RESIDUAL_SMOOTHING= 1) against full reference runs."""
+    g = load_golden(case)
+    mesh = golden_mesh(load_golden(INTEGRATOR_CASES[case]))
+    o = Oracle(mesh, integrator_params(g))
+    r = o.run()
+    assert r["iterations"] == int(g["run_iterations"][0])
+    for k in FIELDS:
+        assert bits_equal(o.state[k], g["run_" + k]), (case, k)
+    assert bits_equal(o.R, g["run_residual"])
+    for k in ("CL", "CD", "CM"):
+        assert bits_equal([r[k]], g["run_" + k]), k
+    rd = g["run_residuals_dat"]
+    assert r["rows"].shape == rd.shape
+    np.testing.assert_allclose(r["rows"], rd, rtol=6e-11, atol=1e-300)
+
+
+def test_oracle_smoothing_matches_two_jacobi_sweeps():
+    """Smoothing pins beyond the fixtures: a zero residual takes one sweep and stays zero; any other
+    residual takes exactly two Jacobi sweeps R <- (R0 + 0.25 sum of interior neighbours) / (1 +
+    0.25 n) (the reference's convergence loop stops after its second pass), checked against an
+    independent numpy evaluation of the two sweeps."""
+    g = load_golden("naca_small")
+    mesh = golden_mesh(g)
+    pd = dict(golden_params(g), residual_smoothing=1)
+    o = Oracle(mesh, pd)
+    o.R[:] = 0.0
+    assert o.smooth() == 1 and np.all(o.R == 0.0)
+    rng = np.random.default_rng(3)
+    R0 = rng.standard_normal((o.n_elem, 5))
+    o.R[:] = R0.ravel()
+    assert o.smooth() == 2
+    start, e2e = mesh["element2faceStart"], mesh["element2element"]
+    nface = np.diff(start)
+    rows = np.repeat(np.arange(o.n_elem), nface)
+    inner = e2e < o.n_elem
+
+    def sweep(x):
+        acc = np.zeros_like(x)
+        np.add.at(acc, rows[inner], 0.25 * x[e2e[inner]])
+        return (R0 + acc) / (1 + 0.25 * nface)[:, None]
+
+    np.testing.assert_allclose(o.R.reshape(-1, 5), sweep(sweep(R0)), rtol=1e-14, atol=1e-15)
