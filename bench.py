@@ -85,6 +85,7 @@ def mixed_precision_leg(mesh, prm, n_cells: int, n_faces: int, steps: int, warmu
     out.update({"unit": "cell-stage/s", "dtype": "f32 flux / f64 update", "iterations": it,
                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": achieved / HBM_PEAK_GBS, "kernel": "kMixStage", "kernel_ms": kern_ms,
+                             "traffic": measured_traffic("kMixStage", n_cells, prm.scheme),
                              "bytes_per_cell_stage": bpc, "launches": launches}})
     return out
 
