@@ -86,7 +86,7 @@ def build_oracle() -> Path:
     out.mkdir(exist_ok=True)
     lib = out / "libe3d_oracle.so"
     tmp = lib.with_suffix(".so.tmp")
-    _run(["gcc", "-std=c99", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-Wall",
+    _run(["gcc", "-std=c99", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", "-Wall",
           str(ROOT / "oracle" / "e3d_oracle.c"), "-o", str(tmp), "-lm"])
     os.replace(tmp, lib)
     return lib

@@ -10,6 +10,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* threads of the parallel loops; 1 runs the sequential restatement (or_set_threads) */
+static int g_threads = 1;
+
 #define E3D_PI 3.141592653589793238
 
 /* FlowField::Initialize, FlowField.cpp:68-101 (rho_inf = p_inf = 1, FlowField.hpp:138-139) */
@@ -260,9 +263,55 @@ void or_face_flux(const or_mesh *m, const or_params *q, const or_state *s, doubl
 	free(gk);
 }
 
+/* Threads of the per-face, per-cell and per-ghost loops (1: the sequential restatement). With
+ * more threads the faces' F*S are formed in parallel and each cell then sums its faces in
+ * ascending face id, the order of the sequential face loop, so every result keeps its bits. */
+void or_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+
+static void residual_parallel(const or_mesh *m, const or_params *q, const or_state *s, double *R,
+                              const unsigned char *gk) {
+	const int nF = m->nFace, nE = m->nElem;
+	double *FS = (double *) malloc(sizeof(double) * 5 * (size_t) nF);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+	for (int f = 0; f < nF; f++) {
+		double r[5];
+		face_flux_one(m, q, s, gk, f, r, NULL);
+		double S = m->faceArea[f];
+		for (int k = 0; k < 5; k++) FS[5 * (size_t) f + k] = r[k] * S;
+	}
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+	for (int e = 0; e < nE; e++) {
+		int b = m->element2faceStart[e], en = m->element2faceStart[e + 1];
+		int fl[16], n = 0;
+		for (int j = b; j < en && n < 16; j++) fl[n++] = m->element2face[j];
+		for (int i = 1; i < n; i++)
+			for (int j = i; j > 0 && fl[j - 1] > fl[j]; j--) {
+				int t = fl[j];
+				fl[j] = fl[j - 1];
+				fl[j - 1] = t;
+			}
+		double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+		for (int i = 0; i < n; i++) {
+			int f = fl[i];
+			int first = m->face2element[2 * (size_t) f] == e;
+			for (int k = 0; k < 5; k++) {
+				if (first) acc[k] += FS[5 * (size_t) f + k];
+				else acc[k] -= FS[5 * (size_t) f + k];
+			}
+		}
+		for (int k = 0; k < 5; k++) R[5 * (size_t) e + k] = acc[k];
+	}
+	free(FS);
+}
+
 /* EulerSolver::computeResidual, EulerSolver.cpp:303-359 (after resetResiduals, :391-395) */
 void or_residual(const or_mesh *m, const or_params *q, const or_state *s, double *R) {
 	unsigned char *gk = ghost_kinds(m);
+	if (g_threads > 1) {
+		residual_parallel(m, q, s, R, gk);
+		free(gk);
+		return;
+	}
 	memset(R, 0, sizeof(double) * 5 * (size_t) m->nElem);
 	for (int f = 0; f < m->nFace; f++) {
 		double r[5];
@@ -280,6 +329,7 @@ void or_residual(const or_mesh *m, const or_params *q, const or_state *s, double
 
 /* TimeStepMethod2, TimeIntegration.cpp:8-40, for every interior cell */
 void or_timestep(const or_mesh *m, const or_params *q, const or_state *s, double *dt) {
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
 	for (int e = 0; e < m->nElem; e++) {
 		double sx = 0, sy = 0, sz = 0;
 		for (int j = m->element2faceStart[e]; j < m->element2faceStart[e + 1]; j++) {
@@ -301,6 +351,7 @@ void or_timestep(const or_mesh *m, const or_params *q, const or_state *s, double
 static void rk_update(const or_mesh *m, const or_params *q, or_state *s, const double *W0, const double *R,
                       const double *dt, double alpha) {
 	double g = q->gamma;
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
 	for (int e = 0; e < m->nElem; e++) {
 		double V = m->cellVolume[e];
 		double d = dt[e];

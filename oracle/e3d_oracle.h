@@ -68,6 +68,9 @@ typedef struct {
 	double *rho, *u, *v, *w, *p, *E, *H, *M; /* nElem + nGhost */
 } or_state;
 
+/* Threads for the parallel loops (default 1: the plain sequential restatement). Results keep
+ * their bits for any count: cells still sum their faces in ascending face id. */
+void or_set_threads(int n);
 void or_freestream(or_params *prm);
 void or_init_state(const or_mesh *m, const or_params *prm, or_state *s);
 void or_update_bc(const or_mesh *m, const or_params *prm, or_state *s);

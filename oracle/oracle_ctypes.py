@@ -55,6 +55,7 @@ def lib():
         L = C.CDLL(str(LIB))
         m, p, s = C.POINTER(OrMesh), C.POINTER(OrParams), C.POINTER(OrState)
         sig = {
+            "or_set_threads": (None, [C.c_int]),
             "or_freestream": (None, [p]),
             "or_init_state": (None, [m, p, s]),
             "or_update_bc": (None, [m, p, s]),
@@ -76,6 +77,11 @@ def lib():
             f.argtypes = a
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's parallel loops (1: sequential); results are bitwise the same."""
+    lib().or_set_threads(int(n))
 
 
 def _i(a):

@@ -109,3 +109,23 @@ def test_oracle_smoothing_matches_two_jacobi_sweeps():
         return (R0 + acc) / (1 + 0.25 * nface)[:, None]
 
     np.testing.assert_allclose(o.R.reshape(-1, 5), sweep(sweep(R0)), rtol=1e-14, atol=1e-15)
+
+
+def test_oracle_threads_keep_every_bit():
+    """The multi-threaded oracle (parallel face fluxes, per-cell ascending-face gather) gives the
+    sequential restatement's bits, so the CPU baseline timed on all cores is the same computation."""
+    import oracle_ctypes
+
+    g = load_golden("bump_small")
+    outs = []
+    for th in (1, 4):
+        oracle_ctypes.set_threads(th)
+        try:
+            o = Oracle(golden_mesh(g), golden_params(g))
+            sums = o.iterate(3)
+            outs.append((sums, {k: o.state[k].copy() for k in FIELDS}, o.R.copy()))
+        finally:
+            oracle_ctypes.set_threads(1)
+    assert bits_equal(outs[0][0], outs[1][0]) and bits_equal(outs[0][2], outs[1][2])
+    for k in FIELDS:
+        assert bits_equal(outs[0][1][k], outs[1][1][k]), k
