@@ -146,15 +146,21 @@ __device__ __forceinline__ void roeFluxEmit(const Prim &L, const Prim &R, double
 	double localC = sqrt(gamma * L.p / L.rho);
 	double delta = (1.0 / 15.0) * localC;
 	double a1 = VT - cT, a5 = VT + cT;
-	double l1 = fabs(a1) > delta ? fabs(a1) : (a1 * a1 + delta * delta) / (2.0 * delta);
-	double l5 = fabs(a5) > delta ? fabs(a5) : (a5 * a5 + delta * delta) / (2.0 * delta);
-	double c2x2 = 2.0 * (cT * cT);
+	// Harten's fix on the acoustic waves; the quotients are formed only where a lane needs them
+	double l1 = fabs(a1), l5 = fabs(a5);
+	if (!(l1 > delta) || !(l5 > delta)) {
+		if (!(l1 > delta)) l1 = (a1 * a1 + delta * delta) / (2.0 * delta);
+		if (!(l5 > delta)) l5 = (a5 * a5 + delta * delta) / (2.0 * delta);
+	}
+	const double cc = cT * cT;
+	double c2x2 = 2.0 * cc;
 	double k1 = dP - rhoT * cT * dVc;
 	double k5 = dP + rhoT * cT * dVc;
-	double kr = (R.rho - L.rho) - dP / (cT * cT);
+	// RN(1 / (2 cc)) scaled by 2 is RN(1 / cc) exactly, so dP / cc shares the reciprocal
+	const double yc = MK ? 1.0 / c2x2 : 0.0;
+	double kr = (R.rho - L.rho) - quot<MK>(dP, cc, 2.0 * yc);
 	double aV = fabs(VT);
 	double rVR = R.rho * VR, lVL = L.rho * VL;
-	const double yc = MK ? 1.0 / c2x2 : 0.0;
 	auto comp = [&](int k, double avg, double e1, double e2, double t2, double e5) {
 		double f1 = quot<MK>(e1 * l1 * k1, c2x2, yc);
 		double f234 = (e2 * kr + t2) * aV;
