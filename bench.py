@@ -214,7 +214,7 @@ def bench_serve(a) -> list[dict]:
         log(f"plain {i}: {cases[i].kind} {cells[i]} cells, {plain[i]['service']:.2f} s, {plain[i]['iterations']} it")
     services = [q["service"] for q in plain]
     lines = []
-    srv = Server(drop=a.drop, max_iter=a.max_iter, hbm_budget=200e9)
+    srv = Server(drop=a.drop, max_iter=a.max_iter, hbm_budget=200e9, admit_workers=a.admit_workers)
     try:
         for lam in a.lambdas:
             arr = poisson_arrivals(len(cases), lam, seed=0)
@@ -231,7 +231,11 @@ def bench_serve(a) -> list[dict]:
                                            f"seed 0), Poisson rate {lam} req/s, {a.drop:g} drop or {a.max_iter} it, "
                                            f"200 GB HBM budget", "rate": lam, "cells_total": sum(cells)},
                     "served": sv, "plain_fp64": pl, "vs_plain_solves_per_h": sv["solves_per_h"] / pl["solves_per_h"],
-                    "max_abs_dCL_vs_fp64": dcl, "max_abs_dCD_vs_fp64": dcd}
+                    "max_abs_dCL_vs_fp64": dcl, "max_abs_dCD_vs_fp64": dcd,
+                    "requests": [{"kind": cases[i].kind, "cells": cells[i], "arrival": round(r["arrival"], 4),
+                                  "admitted": round(r.get("admitted", float("nan")), 4),
+                                  "finished": round(r["finished"], 4), "iterations": r["iterations"],
+                                  "plain_service": round(plain[i]["service"], 4)} for i, r in enumerate(rec)]}
             log(f"rate {lam}: served p50 {sv['p50_s']:.2f} s p99 {sv['p99_s']:.2f} s {sv['solves_per_h']:.0f}/h; "
                 f"plain p50 {pl['p50_s']:.2f} s p99 {pl['p99_s']:.2f} s {pl['solves_per_h']:.0f}/h")
             print(json.dumps(line), flush=True)
@@ -264,6 +268,7 @@ def main() -> None:
     ap.add_argument("--lambdas", type=float, nargs="+", default=[1, 2, 5, 10, 20, 50])
     ap.add_argument("--drop", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=5000)
+    ap.add_argument("--admit-workers", type=int, default=8)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs one resident mesh per GPU and measures one GPU only; "

@@ -25,6 +25,8 @@ namespace {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t) 255; }
 
+constexpr int kMaxClasses = 32; // (scheme, workgroups per CU) pairs
+
 // sizes of one case's device arrays, in the order they are carved from its allocation
 struct CaseLayout {
 	size_t nB, nH, nS, nE, nG, stride, hist, nW, nDefect = 0;
@@ -80,24 +82,27 @@ struct e3d_batch {
 	e3d::MixCase *dCases = nullptr;
 	e3d::MixStatus *dStatus = nullptr;
 	int2 *dBlocks = nullptr;
-	int *dNBlocks = nullptr;
+	int *dNBlocks = nullptr; // one count per launch class
 	size_t blocksCap = 0;
 	// pinned host staging
 	e3d::MixCase *hCases = nullptr;
 	e3d::MixStatus *hStatusInit = nullptr, *hStatus = nullptr;
 	int2 *hBlocks = nullptr;
 	size_t hBlocksCap = 0;
-	int *hN = nullptr;
+	int *hN = nullptr;       // kMaxClasses counts
 	std::vector<CaseHost> cs;
 	std::vector<int> statusInit; // slots whose status must be (re)written at the next activation
 	bool dirty = false;
 	// guards the host-side tables: admissions may run on other threads while iterations run
 	std::mutex mu;
 	// launch description and the captured iteration
-	e3d::MixLaunch L{};
+	// one persistent launch per class of cases: cases are grouped by flux scheme and by how many
+	// of their workgroups fit on a CU (their LDS need), so that a case with large blocks does not
+	// lower the occupancy of every other case
+	std::vector<e3d::MixLaunch> Ls;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
-	e3d::MixLaunch graphL{};
+	std::vector<e3d::MixLaunch> graphLs;
 	bool haveGraph = false;
 	// timing
 	bool timing = false;
@@ -118,8 +123,9 @@ void freeGraph(e3d_batch *b) {
 }
 
 void enqueueIteration(e3d_batch *b) {
-	for (int s = 0; s < 5; s++) e3d::launchMixStage(b->L, s, b->tune, b->stream);
-	e3d::launchMixIterEnd(b->L, b->stream);
+	for (int s = 0; s < 5; s++)
+		for (const auto &L : b->Ls) e3d::launchMixStage(L, s, b->tune, b->stream);
+	e3d::launchMixIterEnd(b->Ls.front(), b->stream);
 }
 
 void capture(e3d_batch *b) {
@@ -128,7 +134,7 @@ void capture(e3d_batch *b) {
 	enqueueIteration(b);
 	hipCheck(hipStreamEndCapture(b->stream, &b->graph), "hipStreamEndCapture");
 	hipCheck(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-	b->graphL = b->L;
+	b->graphLs = b->Ls;
 	b->haveGraph = true;
 }
 
@@ -141,15 +147,35 @@ bool sameLaunch(const e3d::MixLaunch &a, const e3d::MixLaunch &c) {
 void activate(e3d_batch *b) {
 	if (!b->dirty) return;
 	b->sync(); // the pinned staging below may still be read by earlier copies
+	// classes: (scheme, workgroups of the case's own layout that fit on a CU)
+	struct Cls {
+		int scheme, perCU, maxSlots = 2, maxHalo = 0;
+		size_t blocks = 0;
+		std::vector<int> slots;
+	};
+	std::vector<Cls> cls;
 	size_t total = 0;
-	int maxSlots = 2, maxHalo = 0, schemes = 0;
-	for (auto &c : b->cs)
-		if (c.ready) {
-			total += (size_t) c.nBlocks;
-			maxSlots = std::max(maxSlots, c.maxSlots);
-			maxHalo = std::max(maxHalo, c.maxHalo);
-			schemes |= 1 << c.desc.scheme;
+	for (int s = 0; s < b->nSlots; s++) {
+		const CaseHost &c = b->cs[s];
+		if (!c.ready) continue;
+		const int ms = (c.maxSlots + 1) & ~1, cap = e3d::kMixBlock + c.maxHalo;
+		const int fit = e3d::mixResidentPerCU(ms, cap, 1 << c.desc.scheme, b->tune.nontemporal);
+		Cls *k = nullptr;
+		for (auto &x : cls)
+			if (x.scheme == c.desc.scheme && x.perCU == fit) k = &x;
+		if (!k) {
+			if ((int) cls.size() == kMaxClasses) k = &cls.back(); // (cannot happen: 2 schemes x few sizes)
+			else {
+				cls.push_back(Cls{c.desc.scheme, fit});
+				k = &cls.back();
+			}
 		}
+		k->maxSlots = std::max(k->maxSlots, c.maxSlots);
+		k->maxHalo = std::max(k->maxHalo, c.maxHalo);
+		k->blocks += (size_t) c.nBlocks;
+		k->slots.push_back(s);
+		total += (size_t) c.nBlocks;
+	}
 	if (total > b->hBlocksCap) {
 		if (b->hBlocks) (void) hipHostFree(b->hBlocks);
 		b->hBlocksCap = std::max<size_t>(total, 2 * b->hBlocksCap);
@@ -160,19 +186,36 @@ void activate(e3d_batch *b) {
 		b->blocksCap = std::max<size_t>(total, 2 * b->blocksCap);
 		hipCheck(hipMallocAsync((void **) &b->dBlocks, b->blocksCap * sizeof(int2), b->stream), "hipMallocAsync");
 	}
+	for (int s = 0; s < b->nSlots; s++) b->hCases[s] = b->cs[s].ready ? b->cs[s].desc : e3d::MixCase{};
+	std::vector<e3d::MixLaunch> Ls;
 	size_t n = 0;
-	for (int s = 0; s < b->nSlots; s++) {
-		const CaseHost &c = b->cs[s];
-		if (!c.ready) {
-			b->hCases[s] = e3d::MixCase{};
-			continue;
-		}
-		for (int k = 0; k < c.nBlocks; k++) b->hBlocks[n++] = make_int2(s, k);
-		b->hCases[s] = c.desc;
+	const int xc = std::max(1, b->tune.xcds);
+	for (size_t ci = 0; ci < cls.size(); ci++) {
+		const Cls &k = cls[ci];
+		const size_t first = n;
+		for (int s : k.slots)
+			for (int q = 0; q < b->cs[s].nBlocks; q++) b->hBlocks[n++] = make_int2(s, q);
+		b->hN[ci] = (int) (n - first);
+		e3d::MixLaunch L{};
+		L.cases = b->dCases;
+		L.status = b->dStatus;
+		L.blocks = b->dBlocks + first;
+		L.nBlocks = b->dNBlocks + ci;
+		L.nSlots = b->nSlots;
+		L.cap = e3d::kMixBlock + k.maxHalo;
+		L.ms = (k.maxSlots + 1) & ~1;
+		L.schemes = 1 << k.scheme;
+		// never more persistent workgroups than are resident at once
+		const int perCU = std::min(std::max(1, b->tune.wgPerCU), e3d::mixResidentPerCU(L.ms, L.cap, L.schemes,
+		                                                                                b->tune.nontemporal));
+		const int maxGrid = std::max(xc, b->nCU * perCU / xc * xc);
+		L.grid = (int) std::min<size_t>((size_t) maxGrid, (k.blocks + xc - 1) / xc * xc);
+		Ls.push_back(L);
 	}
-	*b->hN = (int) n;
 	if (n) hipCheck(hipMemcpyAsync(b->dBlocks, b->hBlocks, n * sizeof(int2), hipMemcpyHostToDevice, b->stream), "H2D blocks");
-	hipCheck(hipMemcpyAsync(b->dNBlocks, b->hN, sizeof(int), hipMemcpyHostToDevice, b->stream), "H2D nBlocks");
+	if (!cls.empty())
+		hipCheck(hipMemcpyAsync(b->dNBlocks, b->hN, sizeof(int) * cls.size(), hipMemcpyHostToDevice, b->stream),
+		         "H2D nBlocks");
 	hipCheck(hipMemcpyAsync(b->dCases, b->hCases, sizeof(e3d::MixCase) * b->nSlots, hipMemcpyHostToDevice, b->stream),
 	         "H2D cases");
 	for (int s : b->statusInit) {
@@ -182,19 +225,7 @@ void activate(e3d_batch *b) {
 		hipCheck(hipMemcpyAsync(b->dStatus + s, &st, sizeof(st), hipMemcpyHostToDevice, b->stream), "H2D status");
 	}
 	b->statusInit.clear();
-	e3d::MixLaunch L{};
-	L.cases = b->dCases;
-	L.status = b->dStatus;
-	L.blocks = b->dBlocks;
-	L.nBlocks = b->dNBlocks;
-	L.nSlots = b->nSlots;
-	const int xc = std::max(1, b->tune.xcds);
-	const int maxGrid = std::max(xc, b->nCU * std::max(1, b->tune.wgPerCU) / xc * xc);
-	L.grid = (int) std::min<size_t>((size_t) maxGrid, (total + xc - 1) / xc * xc);
-	L.cap = e3d::kMixBlock + maxHalo;
-	L.ms = (maxSlots + 1) & ~1;
-	L.schemes = schemes;
-	b->L = L;
+	b->Ls = std::move(Ls);
 	b->dirty = false;
 }
 
@@ -221,7 +252,7 @@ int e3d_batch_opts_default(e3d_batch_opts *o) {
 	std::memset(o, 0, sizeof(*o));
 	o->hbm_budget = 0.0;
 	o->max_cases = 256;
-	o->wg_per_cu = 4;
+	o->wg_per_cu = 8; // clamped to what fits on a CU at once
 	o->xcds = 8;
 	o->nontemporal = 1;
 	o->use_graph = 1;
@@ -260,16 +291,16 @@ int e3d_batch_create(int device, const e3d_batch_opts *opts, e3d_batch **out) {
 		b->cs.resize(b->nSlots);
 		hipCheck(hipMalloc((void **) &b->dCases, sizeof(e3d::MixCase) * b->nSlots), "hipMalloc");
 		hipCheck(hipMalloc((void **) &b->dStatus, sizeof(e3d::MixStatus) * b->nSlots), "hipMalloc");
-		hipCheck(hipMalloc((void **) &b->dNBlocks, sizeof(int)), "hipMalloc");
+		hipCheck(hipMalloc((void **) &b->dNBlocks, sizeof(int) * kMaxClasses), "hipMalloc");
 		hipCheck(hipHostMalloc((void **) &b->hCases, sizeof(e3d::MixCase) * b->nSlots), "hipHostMalloc");
 		hipCheck(hipHostMalloc((void **) &b->hStatusInit, sizeof(e3d::MixStatus) * b->nSlots), "hipHostMalloc");
 		hipCheck(hipHostMalloc((void **) &b->hStatus, sizeof(e3d::MixStatus) * b->nSlots), "hipHostMalloc");
-		hipCheck(hipHostMalloc((void **) &b->hN, sizeof(int)), "hipHostMalloc");
+		hipCheck(hipHostMalloc((void **) &b->hN, sizeof(int) * kMaxClasses), "hipHostMalloc");
 		std::memset(b->hCases, 0, sizeof(e3d::MixCase) * b->nSlots);
 		std::memset(b->hStatusInit, 0, sizeof(e3d::MixStatus) * b->nSlots);
 		hipCheck(hipMemcpy(b->dStatus, b->hStatusInit, sizeof(e3d::MixStatus) * b->nSlots, hipMemcpyHostToDevice), "H2D");
-		*b->hN = 0;
-		hipCheck(hipMemcpy(b->dNBlocks, b->hN, sizeof(int), hipMemcpyHostToDevice), "H2D");
+		std::fill(b->hN, b->hN + kMaxClasses, 0);
+		hipCheck(hipMemcpy(b->dNBlocks, b->hN, sizeof(int) * kMaxClasses, hipMemcpyHostToDevice), "H2D");
 		*out = b.release();
 		return E3D_OK;
 	});
@@ -518,8 +549,10 @@ int e3d_batch_run(e3d_batch *b, int n) {
 		std::lock_guard<std::mutex> lk(b->mu);
 		hipCheck(hipSetDevice(b->device), "hipSetDevice");
 		activate(b);
-		if (b->L.grid == 0 || n == 0) return E3D_OK;
-		if (b->useGraph && (!b->haveGraph || !sameLaunch(b->graphL, b->L))) capture(b);
+		if (b->Ls.empty() || n == 0) return E3D_OK;
+		bool same = b->haveGraph && b->graphLs.size() == b->Ls.size();
+		for (size_t i = 0; same && i < b->Ls.size(); i++) same = sameLaunch(b->graphLs[i], b->Ls[i]);
+		if (b->useGraph && !same) capture(b);
 		auto ev = [&]() {
 			if (b->evUsed >= b->evPool.size()) {
 				for (int i = 0; i < 64; i++) {
@@ -540,10 +573,10 @@ int e3d_batch_run(e3d_batch *b, int n) {
 				for (int s = 0; s < 5; s++) {
 					hipEvent_t a = b->timing ? ev() : nullptr, z = b->timing ? ev() : nullptr;
 					if (a) hipCheck(hipEventRecord(a, b->stream), "hipEventRecord");
-					e3d::launchMixStage(b->L, s, b->tune, b->stream);
+					for (const auto &L : b->Ls) e3d::launchMixStage(L, s, b->tune, b->stream);
 					if (z) hipCheck(hipEventRecord(z, b->stream), "hipEventRecord");
 				}
-				e3d::launchMixIterEnd(b->L, b->stream);
+				e3d::launchMixIterEnd(b->Ls.front(), b->stream);
 			}
 		}
 		hipCheck(hipGetLastError(), "kernel launch");

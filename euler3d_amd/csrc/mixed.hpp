@@ -92,6 +92,9 @@ struct MixLaunch {
 
 // LDS bytes of a block plan with these extremes
 size_t mixLdsBytes(int maxSlots, int maxHalo);
+// workgroups of the stage kernel that fit on one CU at once with this LDS layout (registers and
+// LDS); a persistent launch larger than that would leave a tail of workgroups running alone
+int mixResidentPerCU(int ms, int cap, int schemes, bool nontemporal);
 // one RK stage (0..4) of every active, unfinished case
 void launchMixStage(const MixLaunch &L, int stage, const MixTuning &t, hipStream_t st);
 // after the 5 stages: per-case RMS, history, convergence, buffer parity
