@@ -562,6 +562,25 @@ void launchMixStage(const MixLaunch &L, int stage, const MixTuning &t, hipStream
 #undef E3D_MIX
 }
 
+int mixResidentPerCU(int ms, int cap, int schemes, bool nontemporal) {
+	const size_t lds = mixLdsBytes(ms, cap - kB);
+	int best = 1 << 30;
+	for (int scheme = 0; scheme < 2; scheme++) {
+		if (!(schemes & (1 << scheme))) continue;
+		int n = 0;
+		hipError_t e;
+		if (scheme == 1)
+			e = nontemporal ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kMixStage<1, true>, kB, lds)
+			                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kMixStage<1, false>, kB, lds);
+		else
+			e = nontemporal ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kMixStage<0, true>, kB, lds)
+			                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kMixStage<0, false>, kB, lds);
+		if (e != hipSuccess || n <= 0) n = 1;
+		best = n < best ? n : best;
+	}
+	return best == (1 << 30) ? 1 : best;
+}
+
 void launchMixIterEnd(const MixLaunch &L, hipStream_t st) {
 	if (L.nSlots <= 0) return;
 	hipLaunchKernelGGL(kMixIterEnd, dim3(L.nSlots), dim3(kEndThreads), 0, st, L.cases, L.status);
