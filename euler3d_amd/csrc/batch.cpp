@@ -14,6 +14,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using e3d_abi::fail;
@@ -43,7 +44,8 @@ struct CaseLayout {
 		    std::max<size_t>(nB, 1) * 8, std::max<size_t>(hist, 1) * 8,               // partial hist
 		    std::max<size_t>(nW, 1) * 4, std::max<size_t>(nW, 1) * 8 * 6,             // wallCell wN wC
 		    8 * 8,                                                                    // scalars
-		    std::max<size_t>(nDefect, 1) * 12};                                       // defect
+		    std::max<size_t>(nDefect, 1) * 12,                                        // defect
+		    std::max<size_t>(nB, 1) * 4};                                             // blockInt
 		total = 0;
 		for (size_t i = 0; i < sizeof(sz) / sizeof(sz[0]); i++) {
 			off[i] = total;
@@ -229,6 +231,49 @@ void activate(e3d_batch *b) {
 	b->dirty = false;
 }
 
+// Lists each block's interior faces first (the stage kernel evaluates only those in its block-wide
+// flux loop and keeps their fluxes in LDS; a boundary face touches one own cell and is formed by
+// that cell's thread). Rewrites the slot arrays and the CSR codes; returns the most interior
+// slots of any block.
+int interiorFirst(e3d::HostPlan &P, std::vector<int> &blockInt, int scheme) {
+	// symmetry faces stay in the flux loop unless the closed form applies (Roe, unit normal as
+	// the kernel sees it in FP32)
+	auto inLoop = [&](size_t s) {
+		if (P.skind[s] == 0) return true;
+		if (P.skind[s] != 2) return false;
+		const float nx = (float) P.snx[s], ny = (float) P.sny[s], nz = (float) P.snz[s];
+		return scheme != 0 || !(std::fabs(nx * nx + ny * ny + nz * nz - 1.0f) < 1e-5f);
+	};
+	blockInt.assign(P.nBlocks, 0);
+	int maxInt = 0;
+	std::vector<int> newIdx;
+	for (int b = 0; b < P.nBlocks; b++) {
+		const int s0 = P.blockSlot[b], s1 = P.blockSlot[b + 1], ns = s1 - s0;
+		newIdx.assign(ns, 0);
+		int k = 0;
+		for (int j = 0; j < ns; j++)
+			if (inLoop(s0 + j)) newIdx[j] = k++;
+		blockInt[b] = k;
+		maxInt = std::max(maxInt, k);
+		for (int j = 0; j < ns; j++)
+			if (!inLoop(s0 + j)) newIdx[j] = k++;
+		auto permute = [&](auto &v) {
+			using T = typename std::decay_t<decltype(v)>::value_type;
+			std::vector<T> tmp(v.begin() + s0, v.begin() + s1);
+			for (int j = 0; j < ns; j++) v[s0 + newIdx[j]] = tmp[j];
+		};
+		permute(P.se0), permute(P.se1), permute(P.skind), permute(P.snx), permute(P.sny), permute(P.snz);
+		permute(P.sS), permute(P.spack);
+		const int c0 = b * P.cellsPerBlock, c1 = std::min(P.nElem, c0 + P.cellsPerBlock);
+		for (size_t i = (size_t) c0 * P.csrStride; i < (size_t) c1 * P.csrStride; i++) {
+			const uint16_t code = P.csr[i];
+			if (code == 0xFFFF) continue;
+			P.csr[i] = (uint16_t) (newIdx[code & 0x7FFF] | (code & 0x8000));
+		}
+	}
+	return maxInt;
+}
+
 void initCaseHost(const e3d::HostPlan &P, const e3d::SolverMesh &sm, const e3d_params &prm, const e3d_case_opts &co,
                   CaseLayout &lay) {
 	lay.nB = (size_t) P.nBlocks;
@@ -360,7 +405,9 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 		hipCheck(hipSetDevice(b->device), "hipSetDevice");
 		e3d::HostPlan P = e3d::buildPlan(sm, e3d::ORDER_MORTON, e3d::kMixBlock);
 		if (P.csrStride > 6) return fail(E3D_ERR_UNSUPPORTED, "cells with more than 6 faces are not on the device path");
-		if (e3d::mixLdsBytes((P.maxSlots + 1) & ~1, P.maxHalo) > 160 * 1024)
+		std::vector<int> blockInt;
+		const int maxInt = interiorFirst(P, blockInt, p->scheme);
+		if (e3d::mixLdsBytes((maxInt + 1) & ~1, P.maxHalo) > 160 * 1024)
 			return fail(E3D_ERR_UNSUPPORTED, "a block of this mesh needs more LDS than a CU has");
 		// closure of every cell: sum over its faces of +-n S, FP64, in reference face order
 		std::vector<float> defect;
@@ -402,7 +449,7 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 		c.nElem = P.nElem;
 		c.nGhost = P.nGhost;
 		c.nBlocks = P.nBlocks;
-		c.maxSlots = P.maxSlots;
+		c.maxSlots = maxInt; // LDS holds the interior slots only
 		c.maxHalo = P.maxHalo;
 		c.prm = *p;
 		fillFree(*p, c.fr);
@@ -458,6 +505,7 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 			c.wC[k] = (double *) at(20) + (3 + k) * std::max<size_t>(lay.nW, 1);
 		}
 		c.scalars = (double *) at(21);
+		up(23, blockInt.data(), blockInt.size() * 4);
 		if (!defect.empty()) up(22, defect.data(), defect.size() * 4);
 		if (lay.nW) {
 			up(19, P.wallCell.data(), lay.nW * 4);
@@ -482,6 +530,7 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 		d.drop = co->drop;
 		d.minResidual = p->min_residual;
 		d.blockSlot = (const int *) at(0);
+		d.blockInt = (const int *) at(23);
 		d.blockHalo = (const int *) at(1);
 		d.haloCell = (const int *) at(2);
 		d.ghostOf = (const int *) at(3);

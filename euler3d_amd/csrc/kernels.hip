@@ -685,7 +685,10 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 			} else {
 				G = mirrorGhost(L.rho, L.u, L.v, L.w, L.p, nx, ny, nz, q.gamma);
 			}
-			storeGhost(out, c, G);
+			// a farfield ghost's state is read by the next stage's classifier; wall and symmetry
+			// ghosts are rebuilt from their cell every stage, so only the iteration's last stage
+			// (the one that forms the RMS partials) stores them, as updateBC leaves them
+			if (kind == 3 || rmsPartial) storeGhost(out, c, G);
 			if (kind == 1) {
 				F[0] = 0;
 				F[1] = nx * G.p;

@@ -262,7 +262,7 @@ template<int SCHEME, bool NT>
 __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double alpha, int cap, int ms, int xcds) {
 	extern __shared__ float lds[];
 	float *ls = lds;            // [6][cap] perturbations of own cells, then halo cells
-	float *lf = lds + 6 * cap;  // [5][ms] (F - F_inf) * S per slot
+	float *lf = lds + 6 * cap;  // [5][ms] (F - F_inf) * S per interior slot
 	const int t = threadIdx.x;
 	const int nb = *(const kConst int *) L.nBlocks;
 	const int per = (nb + (int) gridDim.x - 1) / (int) gridDim.x;
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 		const Free32 f = free32(C);
 		const int c0 = b * kB;
 		const int nc = min(kB, C.nElem - c0);
-		const int s0 = C.blockSlot[b], ns = C.blockSlot[b + 1] - s0;
+		const int s0 = C.blockSlot[b], ni = C.blockInt[b]; // slots, interior ones first
 		const int h0 = C.blockHalo[b], nh = C.blockHalo[b + 1] - h0;
 		const size_t n = (size_t) C.nElem;
 
@@ -308,73 +308,39 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 		}
 		uint32_t pkN = 0;
 		float nxN = 0, nyN = 0, nzN = 0, SN = 0;
-		if (t < ns) {
+		if (t < ni) {
 			pkN = ldS<NT>(C.pack + s0 + t);
 			nxN = ldS<NT>(C.nx + s0 + t), nyN = ldS<NT>(C.ny + s0 + t), nzN = ldS<NT>(C.nz + s0 + t);
 			SN = ldS<NT>(C.S + s0 + t);
 		}
 		__syncthreads();
 
-		// ---- face fluxes of the block into LDS
+		// ---- fluxes of the block's shared faces into LDS. The block's slots list first the faces
+		// evaluated here (interior faces, and symmetry faces without a closed form); the other
+		// boundary faces involve one own cell only and are formed by its thread in the gather.
 		auto ld = [&](int l) -> D {
 			return {ls[l], ls[cap + l], ls[2 * cap + l], ls[3 * cap + l], ls[4 * cap + l], ls[5 * cap + l]};
 		};
-		for (int j = t; j < ns; j += kB) {
+		for (int j = t; j < ni; j += kB) {
 			const int s = s0 + j;
 			const uint32_t pk = pkN;
 			const float nx = nxN, ny = nyN, nz = nzN, S = SN;
-			if (j + kB < ns) {
+			if (j + kB < ni) {
 				pkN = ldS<NT>(C.pack + s + kB);
 				nxN = ldS<NT>(C.nx + s + kB), nyN = ldS<NT>(C.ny + s + kB), nzN = ldS<NT>(C.nz + s + kB);
 				SN = ldS<NT>(C.S + s + kB);
 			}
-			const int l0 = (int) (pk & 0x7FFu), l1 = (int) ((pk >> 11) & 0x7FFu), kind = (int) (pk >> 22);
-			const D Lc = ld(l0);
+			const int l0 = (int) (pk & 0x7FFu), l1 = (int) ((pk >> 11) & 0x7FFu);
 			float F[5];
-			if (kind == 0) {
-				fluxDelta(SCHEME, Lc, ld(l1), nx, ny, nz, f, F);
+			if ((pk >> 22) == 0) {
+				fluxDelta(SCHEME, ld(l0), ld(l1), nx, ny, nz, f, F);
 			} else {
+				// symmetry face without a closed form (AUSM, or a non-planar quad): mirror ghost
+				const D Lc = ld(l0);
 				const int g = C.ghostOf[s];
-				GhostD G;
-				if (kind == 3) {
-					const float Mi = recPtr(in, c0 + l0)[R_M];
-					G = farfieldD(Lc, Mi, recPtr(in, g), nx, ny, nz, f);
-				} else {
-					G = mirrorD(Lc, nx, ny, nz, f);
-				}
-				storeRec(out, g, G.d, G.M);
-				const float Vinf = f.u * nx + f.v * ny + f.w * nz;
-				const float minf = f.r * Vinf;
-				if (kind == 1) {
-					// wall: F = (0, p_g n, 0) (EulerSolver.cpp:331-339)
-					F[0] = -minf;
-					F[1] = G.d.p * nx - minf * f.u;
-					F[2] = G.d.p * ny - minf * f.v;
-					F[3] = G.d.p * nz - minf * f.w;
-					F[4] = -minf * f.h;
-				} else if (kind == 2) {
-					const float n2 = nx * nx + ny * ny + nz * nz;
-					if (SCHEME == 0 && fabsf(n2 - 1.0f) < 1e-5f) {
-						// Roe flux between a state and its mirror image through a unit normal, in
-						// closed form: the Roe-averaged normal velocity vanishes, so only the two
-						// acoustic waves remain and F = (0, (p + rho V^2 + rho c~ V) n, 0) with
-						// c~^2 = c^2 + (gamma - 1) V^2 / 2 (c~ > c_L / 15: no entropy fix)
-						const float rho = f.r + Lc.r;
-						const float V = (f.u + Lc.u) * nx + (f.v + Lc.v) * ny + (f.w + Lc.w) * nz;
-						const float ct = fsqrt(f.g * (f.p + Lc.p) * frcp(rho) + 0.5f * (f.g - 1.0f) * V * V);
-						const float q = Lc.p + rho * V * (V + ct);
-						F[0] = -minf;
-						F[1] = q * nx - minf * f.u;
-						F[2] = q * ny - minf * f.v;
-						F[3] = q * nz - minf * f.w;
-						F[4] = -minf * f.h;
-					} else {
-						fluxDelta(SCHEME, Lc, G.d, nx, ny, nz, f, F);
-					}
-				} else {
-					// farfield: physical flux of the ghost state (Solver::Fc)
-					physDelta(G.d, nx, ny, nz, Vinf, minf, f, F);
-				}
+				const GhostD G = mirrorD(Lc, nx, ny, nz, f);
+				if (stage == 4) storeRec(out, g, G.d, G.M);
+				fluxDelta(SCHEME, Lc, G.d, nx, ny, nz, f, F);
 			}
 #pragma unroll
 			for (int k = 0; k < 5; k++) lf[k * ms + j] = F[k] * S;
@@ -386,14 +352,11 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 		// flux loop (other workgroups on the CU cover the latency).
 		double r0 = 0.0;
 		if (own) {
-			double w0[5];
-#pragma unroll
-			for (int k = 0; k < 5; k++) w0[k] = ldS<NT>(C.W + k * n + e);
-			const float vol = ldS<NT>(C.vol + e), sx = ldS<NT>(C.sx + e), sy = ldS<NT>(C.sy + e), sz = ldS<NT>(C.sz + e);
 			uint16_t code[6];
 			const uint16_t *cl = C.csr + (size_t) e * C.csrStride;
 #pragma unroll
 			for (int i = 0; i < 6; i++) code[i] = i < C.csrStride ? ldS<NT>(cl + i) : (uint16_t) 0xFFFF;
+			const D Lc = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y};
 			double r[5], cmp[5];
 #pragma unroll
 			for (int k = 0; k < 5; k++) r[k] = 0.0, cmp[k] = 0.0;
@@ -403,15 +366,66 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 				if (cd == 0xFFFFu) break;
 				const int j = (int) (cd & 0x7FFFu);
 				const bool neg = (cd & 0x8000u) != 0;
+				float x[5];
+				if (j < ni) {
+#pragma unroll
+					for (int k = 0; k < 5; k++) x[k] = lf[k * ms + j];
+				} else {
+					// boundary face of this cell: ghost state (written for the next stage) and flux
+					const int s = s0 + j;
+					const int kind = (int) (C.pack[s] >> 22);
+					const float nx = C.nx[s], ny = C.ny[s], nz = C.nz[s], S = C.S[s];
+					const int g = C.ghostOf[s];
+					GhostD G = kind == 3 ? farfieldD(Lc, rb.z, recPtr(in, g), nx, ny, nz, f) : mirrorD(Lc, nx, ny, nz, f);
+					// a farfield ghost's state is read by the next stage's classifier; wall and
+					// symmetry ghosts are rebuilt from their cell every stage and only kept for
+					// read-back, so they are stored by the iteration's last stage only
+					if (kind == 3 || stage == 4) storeRec(out, g, G.d, G.M);
+					const float Vinf = f.u * nx + f.v * ny + f.w * nz;
+					const float minf = f.r * Vinf;
+					float F[5];
+					if (kind == 1) {
+						// wall: F = (0, p_g n, 0) (EulerSolver.cpp:331-339)
+						F[0] = -minf;
+						F[1] = G.d.p * nx - minf * f.u;
+						F[2] = G.d.p * ny - minf * f.v;
+						F[3] = G.d.p * nz - minf * f.w;
+						F[4] = -minf * f.h;
+					} else if (kind == 2) {
+						// Roe flux between a state and its mirror image through a unit normal, in
+						// closed form (the other symmetry faces are in the flux loop): the
+						// Roe-averaged normal velocity vanishes, so only the two acoustic waves
+						// remain and F = (0, (p + rho V^2 + rho c~ V) n, 0) with
+						// c~^2 = c^2 + (gamma - 1) V^2 / 2 (c~ > c_L / 15: no entropy fix)
+						const float rho = f.r + Lc.r;
+						const float V = (f.u + Lc.u) * nx + (f.v + Lc.v) * ny + (f.w + Lc.w) * nz;
+						const float ct = fsqrt(f.g * (f.p + Lc.p) * frcp(rho) + 0.5f * (f.g - 1.0f) * V * V);
+						const float q = Lc.p + rho * V * (V + ct);
+						F[0] = -minf;
+						F[1] = q * nx - minf * f.u;
+						F[2] = q * ny - minf * f.v;
+						F[3] = q * nz - minf * f.w;
+						F[4] = -minf * f.h;
+					} else {
+						// farfield: physical flux of the ghost state (Solver::Fc)
+						physDelta(G.d, nx, ny, nz, Vinf, minf, f, F);
+					}
+#pragma unroll
+					for (int k = 0; k < 5; k++) x[k] = F[k] * S;
+				}
 #pragma unroll
 				for (int k = 0; k < 5; k++) {
-					const double x = neg ? -(double) lf[k * ms + j] : (double) lf[k * ms + j];
-					const double y = x - cmp[k];
+					const double xd = neg ? -(double) x[k] : (double) x[k];
+					const double y = xd - cmp[k];
 					const double tt = r[k] + y;
 					cmp[k] = (tt - r[k]) - y;
 					r[k] = tt;
 				}
 			}
+			double w0[5];
+#pragma unroll
+			for (int k = 0; k < 5; k++) w0[k] = ldS<NT>(C.W + k * n + e);
+			const float vol = ldS<NT>(C.vol + e), sx = ldS<NT>(C.sx + e), sy = ldS<NT>(C.sy + e), sz = ldS<NT>(C.sz + e);
 			if (C.defect) {
 				// F_inf through the cell's unclosed remainder (exact freestream flux, FP64)
 				const double cx = C.defect[3 * (size_t) e], cy = C.defect[3 * (size_t) e + 1], cz = C.defect[3 * (size_t) e + 2];

@@ -41,8 +41,9 @@ struct MixCase {
 	int pad0;
 	double drop;        // converged when rms < drop * rms of the first iteration
 	double minResidual; // or when rms < minResidual (absolute)
-	// block plan
-	const int *blockSlot, *blockHalo, *haloCell, *ghostOf;
+	// block plan: each block's slots list its interior faces first (blockInt of them), then its
+	// boundary faces
+	const int *blockSlot, *blockInt, *blockHalo, *haloCell, *ghostOf;
 	const uint32_t *pack;
 	const float *nx, *ny, *nz, *S;
 	const uint16_t *csr;
