@@ -186,3 +186,29 @@ def test_admission_budget_and_backfill():
         with pytest.raises(E3DError) as e:
             b.add(m, naca_params(), drop=0.0, max_iter=10)
         assert e.value.code == E3D_ERR_BUSY
+
+
+def test_server_results_equal_alone_results():
+    """Continuous batching: requests admitted while others run, retired and backfilled, each give
+    bitwise the same coefficients and iteration count as the same case solved alone."""
+    say("[gpu] server vs alone")
+    from euler3d_amd.serve import Server, summarize
+    from euler3d_amd.workloads import poisson_arrivals, request_mix
+
+    cases = request_mix(6, seed=3, min_cells=2e3, max_cells=4e4)
+    built = [c.build() for c in cases]
+    srv = Server(max_cases=3, drop=1e-3, max_iter=300, admit_workers=2, poll_ms=2.0)
+    try:
+        rec = srv.serve([(m, p, c.kind) for (m, p), c in zip(built, cases)], poisson_arrivals(6, 50.0, seed=3))
+    finally:
+        srv.close()
+    s = summarize(rec)
+    assert s["requests"] == 6 and s["p99_s"] >= s["p50_s"] > 0
+    for (m, p), r in zip(built, rec):
+        with Batch() as b:
+            cid = b.add(m, p, drop=1e-3, max_iter=300)
+            b.run(300)
+            st = b.status(cid)
+            alone = b.result(cid)
+        assert r["iterations"] == st["iterations"]
+        assert r["CL"] == alone["CL"] and r["CD"] == alone["CD"], (r, alone["CL"], alone["CD"])

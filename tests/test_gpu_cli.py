@@ -1,0 +1,67 @@
+"""The drop-in executable, `E3D_Solver <case.e3d>`, against the reference's own run of the same
+configuration: residuals.dat rows, the summary.txt coefficients and counts, and the .sol state."""
+from __future__ import annotations
+
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+from conftest import GOLDEN, INTEGRATOR_CASES, ROOT, integrator_params, load_golden, write_par
+
+sys.path.insert(0, str(GOLDEN))
+from make_golden import CONFIG  # noqa: E402  (the .e3d template the fixtures were made with)
+
+pytestmark = pytest.mark.gpu
+
+EXE = ROOT / "euler3d_amd" / "bin" / "E3D_Solver"
+
+
+def summary_values(text: str) -> dict:
+    out = {}
+    for line in text.splitlines():
+        if " = " in line and not line.strip().startswith("["):
+            k, v = line.split(" = ", 1)
+            v = v.strip()
+            if not v.startswith("["):
+                out[k.strip()] = float(v)
+    return out
+
+
+@pytest.mark.parametrize("case", ["naca_small", "bump_small", "naca_euler", "wing_euler_smooth"])
+def test_cli_matches_reference_outputs(case, tmp_path):
+    print(f"[gpu] E3D_Solver case={case}", flush=True)
+    g = load_golden(case)
+    base = INTEGRATOR_CASES.get(case, case)
+    pd = integrator_params(g)
+    p = g["params"]
+    par = write_par(base, tmp_path / "mesh.par")
+    (tmp_path / "out").mkdir()
+    cfg = tmp_path / "case.e3d"
+    rx, ry, rz = pd["ref_point"]
+    cfg.write_text(CONFIG.format(mesh=par.name, par=str(par), out=str(tmp_path / "out"), ocl=int(p[9]), ocd=int(p[10]),
+                                 rx=rx, ry=ry, rz=rz, sref=pd["sref"], cref=pd["cref"], sampling=1, mach=pd["mach"],
+                                 aoa=pd["aoa_deg"], cfl=pd["cfl"], iters=pd["max_iter"], ti=pd["time_integration"],
+                                 smooth=pd["residual_smoothing"]))
+    r = subprocess.run([str(EXE), str(cfg)], cwd=str(tmp_path), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    # residuals.dat: header + iter rms Cx Cy Cz CMx CMy CMz, %.10e
+    rows = np.loadtxt(tmp_path / "out" / "residuals.dat", skiprows=1, ndmin=2)
+    rd = g["run_residuals_dat"]
+    assert rows.shape == rd.shape
+    np.testing.assert_array_equal(rows[:, 0], rd[:, 0])
+    np.testing.assert_allclose(rows[:, 1:], rd[:, 1:], rtol=6e-11, atol=1e-13)
+    # summary.txt (timings differ by construction)
+    ours = summary_values((tmp_path / "out" / "summary.txt").read_text())
+    ref = summary_values(bytes(g["run_summary_txt"]).decode())
+    assert ours["Iterations"] == ref["Iterations"] and ours["NElem"] == ref["NElem"] and ours["Ncores"] == 1
+    for k in ("CL", "CD", "CM"):
+        assert abs(ours[k] - ref[k]) <= 1e-12 * max(abs(ref[k]), 1e-3), (k, ours[k], ref[k])
+    # <par>.sol: the interior primitives in the writer's 6-digit scientific format
+    sol = np.loadtxt(str(par) + ".sol", skiprows=1, ndmin=2)
+    n = int(ref["NElem"])
+    assert sol.shape == (n, 6)
+    for col, k in enumerate(("rho", "u", "v", "w", "p", "E")):
+        expect = np.array([float("%e" % x) for x in g["run_" + k][:n]])
+        np.testing.assert_array_equal(sol[:, col], expect, err_msg=k)
+    assert (str(par) + ".bry") and (tmp_path / "out" / "cp_part_0.dat").exists()
