@@ -785,6 +785,81 @@ int e3d_run(e3d_ctx *c, double *rows, int max_rows, int *n_rows, int *iters, dou
 	});
 }
 
+namespace {
+constexpr char kCkptMagic[8] = {'E', '3', 'D', 'C', 'K', 'P', 'T', '1'};
+struct CkptHeader {
+	char magic[8];
+	int32_t nElem, nGhost, nFace, machDerived;
+	int32_t timeIntegration, scheme, pad[2];
+};
+}// namespace
+
+int e3d_checkpoint_write(e3d_ctx *c, const char *path) {
+	if (!c || !path) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		const size_t nT = (size_t) c->nTot, nE = (size_t) c->nElem;
+		std::vector<double> st(nT * e3d::kStateWidth), w(5 * nE);
+		hipCheck(hipMemcpyAsync(st.data(), c->st[c->cur].p, st.size() * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+		hipCheck(hipMemcpyAsync(w.data(), c->W0.p, w.size() * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+		c->sync();
+		CkptHeader h{};
+		std::memcpy(h.magic, kCkptMagic, 8);
+		h.nElem = c->nElem, h.nGhost = c->nGhost, h.nFace = c->nFace, h.machDerived = c->machDerived;
+		h.timeIntegration = c->prm.time_integration, h.scheme = c->prm.scheme;
+		std::FILE *f = std::fopen(path, "wb");
+		if (!f) return fail(E3D_ERR_IO, std::string("cannot open ") + path);
+		bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
+		std::vector<double> col(nT);
+		for (int k = 0; k < e3d::kStateWidth && ok; k++) {
+			toReference(c, st, k, col.data());
+			ok = std::fwrite(col.data(), 8, nT, f) == nT;
+		}
+		std::vector<double> wr(nE);
+		for (int k = 0; k < 5 && ok; k++) {
+			for (size_t i = 0; i < nE; i++) wr[c->new2old[i]] = w[k * nE + i];
+			ok = std::fwrite(wr.data(), 8, nE, f) == nE;
+		}
+		ok = (std::fclose(f) == 0) && ok;
+		if (!ok) return fail(E3D_ERR_IO, std::string("writing ") + path);
+		return E3D_OK;
+	});
+}
+
+int e3d_checkpoint_read(e3d_ctx *c, const char *path) {
+	if (!c || !path) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		std::FILE *f = std::fopen(path, "rb");
+		if (!f) return fail(E3D_ERR_IO, std::string("cannot open ") + path);
+		CkptHeader h{};
+		const size_t nT = (size_t) c->nTot, nE = (size_t) c->nElem;
+		bool ok = std::fread(&h, sizeof(h), 1, f) == 1;
+		if (ok && (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.nElem != c->nElem || h.nGhost != c->nGhost ||
+		           h.nFace != c->nFace)) {
+			std::fclose(f);
+			return fail(E3D_ERR_ARG, std::string(path) + " is not a checkpoint of this mesh");
+		}
+		std::vector<double> st(nT * e3d::kStateWidth), col(nT), w(5 * nE), wr(nE);
+		for (int k = 0; k < e3d::kStateWidth && ok; k++) {
+			ok = std::fread(col.data(), 8, nT, f) == nT;
+			if (ok) toDevice(c, col.data(), k, st);
+		}
+		for (int k = 0; k < 5 && ok; k++) {
+			ok = std::fread(wr.data(), 8, nE, f) == nE;
+			if (ok)
+				for (size_t i = 0; i < nE; i++) w[k * nE + i] = wr[c->new2old[i]];
+		}
+		std::fclose(f);
+		if (!ok) return fail(E3D_ERR_IO, std::string("truncated checkpoint ") + path);
+		c->st[c->cur].upload(st.data(), st.size(), c->stream);
+		c->W0.upload(w.data(), w.size(), c->stream);
+		c->sync();
+		c->machDerived = h.machDerived;
+		return E3D_OK;
+	});
+}
+
 int e3d_last_elapsed_ms(e3d_ctx *c, double *ms) {
 	if (!c || !ms) return fail(E3D_ERR_ARG, "null argument");
 	*ms = c->lastMs;

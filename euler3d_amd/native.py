@@ -83,6 +83,8 @@ _SIGS = {
     "e3d_aero": (C.c_int, [P, DP, DP]),
     "e3d_run": (C.c_int, [P, DP, C.c_int, IP, IP, DP, DP, DP]),
     "e3d_last_elapsed_ms": (C.c_int, [P, DP]),
+    "e3d_checkpoint_write": (C.c_int, [P, C.c_char_p]),
+    "e3d_checkpoint_read": (C.c_int, [P, C.c_char_p]),
     "e3d_set_kernel_timing": (C.c_int, [P, C.c_int]),
     "e3d_kernel_timing": (C.c_int, [P, DP, IP, C.c_char_p, C.c_int]),
     "e3d_destroy": (None, [P]),

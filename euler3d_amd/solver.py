@@ -201,6 +201,13 @@ class EulerSolver:
         return {"rows": rows[: 8 * k].reshape(k, 8), "iterations": iters.value, "CL": coeffs[0], "CD": coeffs[1],
                 "CM": coeffs[2], "force_coeff": cf, "moment_coeff": cm}
 
+    def checkpoint_write(self, path: str) -> None:
+        """Exact FP64 checkpoint (state, explicit Euler conservative state, Mach bookkeeping)."""
+        N.check(N.lib().e3d_checkpoint_write(self._h, str(path).encode()))
+
+    def checkpoint_read(self, path: str) -> None:
+        N.check(N.lib().e3d_checkpoint_read(self._h, str(path).encode()))
+
     def elapsed_ms(self) -> float:
         ms = C.c_double()
         N.check(N.lib().e3d_last_elapsed_ms(self._h, C.byref(ms)))

@@ -139,6 +139,13 @@ int e3d_aero(e3d_ctx *c, double cf[3], double cm[3]);
  * holds them. coeffs = CL, CD, CM after the loop. */
 int e3d_run(e3d_ctx *c, double *rows, int max_rows, int *n_rows, int *iters, double coeffs[3], double cf[3],
             double cm[3]);
+/* Exact FP64 checkpoint of a context's solution (every cell's primitives including ghosts, the
+ * explicit Euler step's conservative state, and whether the stored interior Mach numbers are
+ * current), in reference cell numbering. Reading it into a context of the same mesh and
+ * parameters continues bit for bit as if the run had not been interrupted. No counterpart in the
+ * reference (restart files are not part of Euler3D). */
+int e3d_checkpoint_write(e3d_ctx *c, const char *path);
+int e3d_checkpoint_read(e3d_ctx *c, const char *path);
 /* device time of the last e3d_iterate / e3d_run call in milliseconds (HIP events) */
 int e3d_last_elapsed_ms(e3d_ctx *c, double *ms);
 /* Per-launch timing of the dominant (per-stage residual + update) kernel: when enabled, every

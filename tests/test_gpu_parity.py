@@ -213,3 +213,35 @@ def test_gpu_other_integrators_match_reference(case, path, tmp_path, monkeypatch
     for k in ("CL", "CD", "CM"):
         ref = float(g["run_" + k][0])
         assert abs(r[k] - ref) <= 1e-12 * max(abs(ref), 1e-300), (k, r[k], ref)
+
+
+@pytest.mark.parametrize("case", ["naca_small", "naca_euler"])
+def test_gpu_checkpoint_resume_is_exact(case, tmp_path):
+    """Run n iterations, or n/2, checkpoint, read into a fresh context, n/2 more: same bits."""
+    say(f"[gpu] checkpoint case={case}")
+    from euler3d_amd import E3DError
+
+    g = load_golden(case)
+    base = INTEGRATOR_CASES.get(case, case)
+    pd = integrator_params(g)
+    m = Mesh.read(str(write_par(base, tmp_path / f"{base}.par"))).preprocess(pd["cref"])
+    with EulerSolver(m, native_params(pd)) as s:
+        full = s.iterate(30)
+        ref = s.get_state()
+    ck = tmp_path / "state.ckpt"
+    with EulerSolver(m, native_params(pd)) as s:
+        first = s.iterate(15)
+        s.checkpoint_write(str(ck))
+    with EulerSolver(m, native_params(pd)) as s:
+        s.checkpoint_read(str(ck))
+        second = s.iterate(15)
+        st = s.get_state()
+        with pytest.raises(E3DError):
+            s.checkpoint_read(str(tmp_path / "missing.ckpt"))
+    assert bits_equal(np.concatenate([first, second]), full)
+    for k in FIELDS:
+        assert bits_equal(st[k], ref[k]), k
+    other = Mesh.box(3, 3, 3).preprocess(1.0)
+    with EulerSolver(other, native_params(pd)) as s:
+        with pytest.raises(E3DError, match="not a checkpoint of this mesh"):
+            s.checkpoint_read(str(ck))
