@@ -1,11 +1,12 @@
 """Build recipe of the native parts: libe3d_hip.so (HIP kernels for gfx950 + host C++), the
-E3D_Solver-compatible executable, and the CPU oracle used by the tests.
+E3D_Solver- and E3D_POST-compatible executables, and the CPU oracle used by the tests.
 
 Run as ``python -m euler3d_amd.build`` or through ``__graft_entry__.build()``. Outputs go next to
 the sources (in-tree) so that they travel with the repository snapshot:
 
     euler3d_amd/lib/libe3d_hip.so
     euler3d_amd/bin/E3D_Solver
+    euler3d_amd/bin/E3D_POST
     oracle/lib/libe3d_oracle.so          (test infrastructure)
 
 Host C++ is compiled with g++ -O2 -ffp-contract=off (the arithmetic of the mesh metrics must
@@ -28,7 +29,9 @@ OBJ = ROOT / "_build" / "obj"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 ARCH = os.environ.get("E3D_OFFLOAD_ARCH", "gfx950")
 
-HOST_SRCS = ["mesh_io.cpp", "config.cpp", "preprocess.cpp", "generators.cpp", "plan.cpp", "capi.cpp", "batch.cpp"]
+HOST_SRCS = ["mesh_io.cpp", "config.cpp", "preprocess.cpp", "generators.cpp", "plan.cpp", "post.cpp", "capi.cpp",
+             "batch.cpp"]
+MAINS = {"E3D_Solver": "solver_main.cpp", "E3D_POST": "post_main.cpp"}
 DEVICE_SRCS = ["kernels.hip", "kernels_mixed.hip"]
 
 CXXFLAGS = ["-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
@@ -65,8 +68,8 @@ def build_native(jobs: int = 8) -> dict[str, Path]:
         o = OBJ / (s + ".o")
         steps.append([hipcc, *HIPFLAGS, "-c", str(CSRC / s), "-o", str(o)])
         objs.append(o)
-    main_o = OBJ / "solver_main.cpp.o"
-    steps.append(["g++", *CXXFLAGS, "-c", str(CSRC / "solver_main.cpp"), "-o", str(main_o)])
+    for src in MAINS.values():
+        steps.append(["g++", *CXXFLAGS, "-c", str(CSRC / src), "-o", str(OBJ / (src + ".o"))])
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         list(ex.map(_run, steps))
     lib = PKG / "lib" / "libe3d_hip.so"
@@ -74,10 +77,13 @@ def build_native(jobs: int = 8) -> dict[str, Path]:
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp),
           f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"])
     os.replace(tmp, lib)
-    exe = PKG / "bin" / "E3D_Solver"
-    _run(["g++", str(main_o), "-o", str(exe), f"-L{PKG}/lib", "-le3d_hip",
-          "-Wl,-rpath,$ORIGIN/../lib", f"-Wl,-rpath,{ROCM}/lib"])
-    return {"lib": lib, "exe": exe}
+    out = {"lib": lib}
+    for name, src in MAINS.items():
+        exe = PKG / "bin" / name
+        _run(["g++", str(OBJ / (src + ".o")), "-o", str(exe), f"-L{PKG}/lib", "-le3d_hip",
+              "-Wl,-rpath,$ORIGIN/../lib", f"-Wl,-rpath,{ROCM}/lib"])
+        out[name] = exe
+    return out
 
 
 def build_oracle() -> Path:

@@ -283,7 +283,7 @@ int e3d_mesh_read(const char *path, int require_par, e3d_mesh **out) {
 	return guarded([&] {
 		auto m = std::make_unique<e3d_mesh>();
 		try {
-			m->file = e3d::readSu2(path, require_par != 0);
+			m->file = e3d::readSu2(path, require_par ? e3d::ParBlock::Required : e3d::ParBlock::Optional);
 		} catch (const std::exception &e) {
 			return fail(E3D_ERR_IO, e.what());
 		}
@@ -297,6 +297,41 @@ int e3d_mesh_write(const e3d_mesh *m, const char *path, int precision, int par_b
 	return guarded([&] {
 		e3d::writeSu2(m->file, path, precision, par_block != 0);
 		return E3D_OK;
+	});
+}
+
+int e3d_post_tecplot_files(const char *out_path, int n_part, const char *const *par_paths,
+                           const char *const *sol_paths) {
+	if (!out_path || n_part < 0 || (n_part > 0 && (!par_paths || !sol_paths))) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		std::vector<std::string> par, sol;
+		for (int i = 0; i < n_part; i++) {
+			if (!par_paths[i] || !sol_paths[i]) return fail(E3D_ERR_ARG, "null path");
+			par.emplace_back(par_paths[i]);
+			sol.emplace_back(sol_paths[i]);
+		}
+		try {
+			e3d::writeTecplot(out_path, par, sol);
+		} catch (const std::invalid_argument &) {
+			throw;
+		} catch (const std::exception &e) {
+			return fail(E3D_ERR_IO, e.what());
+		}
+		return E3D_OK;
+	});
+}
+
+int e3d_post_tecplot(const char *config_path) {
+	if (!config_path) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		e3d::Params p = e3d::readConfig(config_path);
+		std::vector<std::string> sol = e3d::postSolutionPaths(p);
+		std::vector<const char *> pp, sp;
+		for (size_t i = 0; i < sol.size(); i++) {
+			pp.push_back(p.partitionFiles[i].c_str());
+			sp.push_back(sol[i].c_str());
+		}
+		return e3d_post_tecplot_files(p.outputFile.c_str(), (int) pp.size(), pp.data(), sp.data());
 	});
 }
 

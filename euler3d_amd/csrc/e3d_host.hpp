@@ -36,6 +36,7 @@ struct MeshFile {
 	std::vector<int> elemNodes;
 	std::vector<Marker> markers;
 	int nInternalMarker = 0;       // .par NINTERNAL_MARKER (must be 0 here: one partition)
+	long nElemDeclared = 0;        // NELEM of the file, unknown VTK ids included
 	int nNode() const { return (int) xyz.size() / 3; }
 	int nElem() const { return (int) elemVtk.size(); }
 	int nBoundaryElems() const {
@@ -112,11 +113,20 @@ struct Params {
 };
 
 // ---- mesh_io.cpp
-MeshFile readSu2(const std::string &path, bool requireParBlock);
+// parBlock: the reader's handling of the .par NINTERNAL_MARKER block
+enum class ParBlock { Ignore, Optional, Required };
+MeshFile readSu2(const std::string &path, ParBlock parBlock);
 void writeSu2(const MeshFile &m, const std::string &path, int precision, bool parBlock);
 
 // ---- config.cpp
 Params readConfig(const std::string &path);
+
+// ---- post.cpp
+// Tecplot ASCII file of the given partitions and their .sol files (src/post/post.cpp:49-133).
+void writeTecplot(const std::string &outPath, const std::vector<std::string> &parPaths,
+                  const std::vector<std::string> &solPaths);
+// the .sol files Post::Post reads: <dir of OUTPUT_FILE>/<partition file name>.sol (post.cpp:18-30)
+std::vector<std::string> postSolutionPaths(const Params &p);
 
 // ---- preprocess.cpp
 // Builds SolverMesh from a MeshFile. O(N log N) in the number of cells; numbering, orientation

@@ -105,7 +105,7 @@ int lineInts(const char *b, const char *e, long *out, int max) {
 
 }// namespace
 
-MeshFile readSu2(const std::string &path, bool requireParBlock) {
+MeshFile readSu2(const std::string &path, ParBlock parBlock) {
 	std::ifstream in(path, std::ios::binary);
 	if (!in) throw std::runtime_error("Error while opening mesh file ! " + path);
 	std::string buf((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
@@ -118,6 +118,7 @@ MeshFile readSu2(const std::string &path, bool requireParBlock) {
 
 	if (!c.findLine("NELEM=", b, e)) throw std::runtime_error("Dimension keyword not found \"NELEM=\" !");
 	long nElem = parseLongAfter(b, e, 6);
+	m.nElemDeclared = nElem;
 	m.elemVtk.reserve(nElem);
 	m.elemStart.reserve(nElem + 1);
 	m.elemStart.push_back(0);
@@ -172,12 +173,13 @@ MeshFile readSu2(const std::string &path, bool requireParBlock) {
 		m.markers.push_back(std::move(mk));
 	}
 
+	if (parBlock == ParBlock::Ignore) return m; // SU2MeshParser reads no further
 	Cursor c2 = c;
 	if (c2.findLine("NINTERNAL_MARKER=", b, e)) {
 		m.nInternalMarker = (int) parseLongAfter(b, e, 17);
 		if (m.nInternalMarker != 0)
 			throw std::runtime_error("NINTERNAL_MARKER > 0: multi-partition .par input is not supported (one resident mesh per device)");
-	} else if (requireParBlock) {
+	} else if (parBlock == ParBlock::Required) {
 		throw std::runtime_error("MPI Boundary Elements keyword \"NINTERNAL_MARKER=\" not found !");
 	}
 	return m;

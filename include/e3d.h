@@ -19,6 +19,7 @@
  *   SU2MeshParser / MeshPartition src/parser/SU2MeshParser.cpp:12-240,
  *                                 src/parser/MeshPartition.cpp:15-112  -> e3d_mesh_read
  *   SU2Writer                     src/partition/SU2Writer.cpp:17-123   -> e3d_mesh_write
+ *   Post(config).Write()          src/post/post.cpp:18-133             -> e3d_post_tecplot
  *   Mesh::solveConnectivity + Metrics  include/mesh/Mesh.hpp:36-105,
  *                                 src/mesh/Metrics.cpp:9-56            -> e3d_mesh_preprocess
  *
@@ -94,6 +95,15 @@ int e3d_mesh_read(const char *path, int require_par, e3d_mesh **out);
 /* SU2Writer layout; precision = digits after the point of the %e coordinates; par_block adds
  * "NINTERNAL_MARKER= 0" so the file is a one-partition .par */
 int e3d_mesh_write(const e3d_mesh *m, const char *path, int precision, int par_block);
+/* Post::Post(config) + Post::Write()  src/post/post.cpp:18-133 (the E3D_POST executable,
+ * src/postprocessor.cpp:17-40): writes the Tecplot ASCII file OUTPUT_FILE of the .e3d file, one
+ * FEBRICK zone per listed partition, with the cell values read from
+ * <directory of OUTPUT_FILE>/<partition file name>.sol. Host only: no device is used.
+ * Missing or unreadable inputs give E3D_ERR_IO (the reference exits the process). */
+int e3d_post_tecplot(const char *config_path);
+/* The same writer over explicit paths: n_part partition meshes and the .sol file of each. */
+int e3d_post_tecplot_files(const char *out_path, int n_part, const char *const *par_paths,
+                           const char *const *sol_paths);
 /* Synthetic meshes. tags6: six comma-separated tags for the x-, x+, y-, y+, z-, z+ sides. */
 int e3d_mesh_gen_box(int nx, int ny, int nz, double lx, double ly, double lz, const char *tags6, e3d_mesh **out);
 int e3d_mesh_gen_naca(int ni, int nj, int nk, double thickness, double camber, double camber_pos,

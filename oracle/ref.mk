@@ -1,5 +1,5 @@
-# Builds the reference Euler3D solver (E3D_Solver) and the golden-vector harness from the
-# reference sources where they lie, into oracle/_ref/. TEST INFRASTRUCTURE ONLY: this runs only
+# Builds the reference Euler3D solver (E3D_Solver), its Tecplot post-processor (E3D_POST) and the
+# golden-vector harness from the reference sources where they lie, into oracle/_ref/. TEST INFRASTRUCTURE ONLY: this runs only
 # where the reference tree is present; the product never links or loads what it builds.
 #
 # Flags: -O2 -ffp-contract=off (IEEE-strict). The reference's own Release build uses -Ofast
@@ -23,7 +23,11 @@ SOLVER_SRCS := $(wildcard $(REF)/src/solver/*.cpp) $(wildcard $(REF)/src/mesh/*.
                $(REF)/src/parallelization/MPIHandler.cpp
 OBJS := $(patsubst $(REF)/src/%.cpp,$(OUT)/obj/%.o,$(SOLVER_SRCS))
 
-all: $(OUT)/E3D_Solver $(OUT)/ref_harness
+POST_SRCS := $(REF)/src/post/post.cpp $(REF)/src/parser/SolutionPost.cpp $(REF)/src/utils/Logger.cpp
+POST_OBJS := $(patsubst $(REF)/src/%.cpp,$(OUT)/obj/%.o,$(POST_SRCS)) $(OUT)/obj/parser/SU2MeshParser.o \
+             $(OUT)/obj/parser/SimConfig.o
+
+all: $(OUT)/E3D_Solver $(OUT)/ref_harness $(OUT)/E3D_POST
 
 $(MPILIB)/.stamp:
 	mkdir -p $(MPILIB)
@@ -38,6 +42,13 @@ $(OUT)/obj/%.o: $(REF)/src/%.cpp
 $(OUT)/obj/Solver_main.o: $(REF)/src/Solver.cpp
 	@mkdir -p $(dir $@)
 	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/obj/Post_main.o: $(REF)/src/postprocessor.cpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/E3D_POST: $(POST_OBJS) $(OUT)/obj/Post_main.o $(MPILIB)/.stamp
+	$(CXX) -o $@ $(POST_OBJS) $(OUT)/obj/Post_main.o $(LDFLAGS) -lpthread
 
 $(OUT)/obj/ref_harness.o: oracle/ref_harness.cpp
 	@mkdir -p $(dir $@)

@@ -1,7 +1,10 @@
 """The drop-in executable, `E3D_Solver <case.e3d>`, against the reference's own run of the same
-configuration: residuals.dat rows, the summary.txt coefficients and counts, and the .sol state."""
+configuration: residuals.dat rows, the summary.txt coefficients and counts, and the .sol state;
+then `E3D_POST <case.e3d>` on that run's output."""
 from __future__ import annotations
 
+import re
+import shutil
 import subprocess
 import sys
 
@@ -15,6 +18,7 @@ from make_golden import CONFIG  # noqa: E402  (the .e3d template the fixtures we
 pytestmark = pytest.mark.gpu
 
 EXE = ROOT / "euler3d_amd" / "bin" / "E3D_Solver"
+POST = ROOT / "euler3d_amd" / "bin" / "E3D_POST"
 
 
 def summary_values(text: str) -> dict:
@@ -65,3 +69,14 @@ def test_cli_matches_reference_outputs(case, tmp_path):
         expect = np.array([float("%e" % x) for x in g["run_" + k][:n]])
         np.testing.assert_array_equal(sol[:, col], expect, err_msg=k)
     assert (str(par) + ".bry") and (tmp_path / "out" / "cp_part_0.dat").exists()
+    # E3D_POST on the run: it reads <dir of OUTPUT_FILE>/<partition name>.sol, and its cell values
+    # are the .sol numbers in the Tecplot block layout
+    shutil.copyfile(str(par) + ".sol", tmp_path / "out" / (par.name + ".sol"))
+    r = subprocess.run([str(POST), str(cfg)], cwd=str(tmp_path), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = (tmp_path / "out" / "out.dat").read_text().splitlines()
+    nn = int(re.search(r"Nodes=(\d+), Elements=(\d+)", lines[2]).group(1))
+    assert int(re.search(r"Elements=(\d+)", lines[2]).group(1)) == n
+    vals = np.array(lines[4 + 3 * nn:4 + 3 * nn + 6 * n], dtype=np.float64).reshape(6, n).T
+    np.testing.assert_array_equal(vals, sol)
+    assert len(lines) == 4 + 3 * nn + 7 * n
