@@ -29,8 +29,8 @@ OBJ = ROOT / "_build" / "obj"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 ARCH = os.environ.get("E3D_OFFLOAD_ARCH", "gfx950")
 
-HOST_SRCS = ["mesh_io.cpp", "config.cpp", "preprocess.cpp", "generators.cpp", "plan.cpp", "post.cpp", "capi.cpp",
-             "batch.cpp"]
+HOST_SRCS = ["mesh_io.cpp", "config.cpp", "preprocess.cpp", "partitions.cpp", "generators.cpp", "plan.cpp", "post.cpp",
+             "capi.cpp", "batch.cpp"]
 MAINS = {"E3D_Solver": "solver_main.cpp", "E3D_POST": "post_main.cpp"}
 DEVICE_SRCS = ["kernels.hip", "kernels_mixed.hip"]
 

@@ -16,6 +16,10 @@ struct e3d_mesh {
 	e3d::MeshFile file;
 	e3d::SolverMesh sm;
 	bool preprocessed = false;
+	// several partitions read together (e3d_mesh_read_partitions): their files and, once
+	// preprocessed, where each sits in the composite (file is then their merged copy)
+	std::vector<e3d::MeshFile> parts;
+	std::vector<e3d::PartRange> ranges;
 };
 
 namespace e3d_abi {

@@ -381,6 +381,8 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 		return fail(E3D_ERR_UNSUPPORTED, "the mixed-precision engine runs RK5 (TIME_INTEGRATION= 2) without residual "
 		                                 "smoothing; use e3d_create for the other integrators");
 	if (co->max_iter <= 0 || co->hist_cap < 0 || co->drop < 0) return fail(E3D_ERR_ARG, "invalid case options");
+	if (!m->parts.empty())
+		return fail(E3D_ERR_UNSUPPORTED, "the mixed-precision engine takes single meshes; use e3d_create for partitioned input");
 	const e3d::SolverMesh &sm = m->sm;
 	if (sm.nElem <= 0) return fail(E3D_ERR_ARG, "empty mesh");
 	if ((long long) sm.nElem + sm.nGhost >= (1LL << 31) / e3d::kRecWidth) return fail(E3D_ERR_ARG, "mesh too large");

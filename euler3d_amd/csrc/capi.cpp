@@ -90,13 +90,19 @@ struct e3d_ctx {
 	int nbrStride = 0;
 	DBuf<int> nbr;
 	DBuf<uint8_t> nface;
-	DBuf<double> R0, R1, R2, blockMax;
-	DBuf<int> smoothFlag;
+	DBuf<double> R0, R1, R2, blockMax, cellMax;
+	DBuf<int> smoothFlag, partStart;
 	// aero (wall list order)
 	int nWall = 0;
 	DBuf<int> wallCell;
 	DBuf<double> wN[3], wC[3];
 	double cpFactor = 0.0;
+	// partitions of a composite mesh (one entry otherwise): wall ranges for the per-partition
+	// coefficients, interface ghosts (reference ids) and the cells they copy
+	int nPart = 1;
+	std::vector<int> wall0, nWallPart;
+	std::vector<int> mpiGhost, mpiCell;
+	DBuf<double> aeroOut;
 
 	e3d::DevState state(int which) { return {st[which].p}; }
 	e3d::DevState state() { return state(cur); }
@@ -162,7 +168,7 @@ void runStage(e3d_ctx *c, int stage, double *rmsPartial) {
 		if (smooth) {
 			e3d::launchGather(c->dm, c->fs.p, c->R0.p, c->stream);
 			e3d::launchSmooth(c->R0.p, c->R1.p, c->R2.p, c->nbr.p, c->nface.p, c->nbrStride, c->nElem, c->blockMax.p,
-			                  c->smoothFlag.p, c->stream);
+			                  c->smoothFlag.p, c->partStart.p, c->nPart, c->cellMax.p, c->stream);
 			Rin = c->R2.p;
 		}
 		const int integ = ti == 2 ? 0 : ti == 0 ? 1 : 2;
@@ -183,10 +189,10 @@ const char *stageKernelName(const e3d_ctx *c) {
 // NaN in the residual smoothing (the reference exits the process, ResidualVar.hpp:96-104)
 void checkSmoothing(e3d_ctx *c) {
 	if (!c->prm.residual_smoothing) return;
-	int f[2] = {0, 0};
-	hipCheck(hipMemcpyAsync(f, c->smoothFlag.p, sizeof(f), hipMemcpyDeviceToHost, c->stream), "D2H flag");
+	int nan = 0;
+	hipCheck(hipMemcpyAsync(&nan, c->smoothFlag.p + c->nPart, sizeof(int), hipMemcpyDeviceToHost, c->stream), "D2H flag");
 	c->sync();
-	if (f[1]) throw std::domain_error("NaN residual in the residual smoothing");
+	if (nan) throw std::domain_error("NaN residual in the residual smoothing");
 }
 
 // host reference-order field k <-> device state (interior permuted, ghosts as they are, 8 doubles
@@ -265,8 +271,6 @@ int e3d_config_read(const char *path, e3d_params *out, char *mesh_path, int mesh
 		out->sampling = p.sampling;
 		out->min_residual = p.minResidual;
 		out->min_aerocoeff_error = p.minAeroCoeffError;
-		if (p.nPartition != 1)
-			return fail(E3D_ERR_UNSUPPORTED, "PARTITION_FILES must be 1: the whole mesh is resident on one device");
 		auto copy = [](const std::string &s, char *dst, int len) {
 			if (!dst || len <= 0) return;
 			std::strncpy(dst, s.c_str(), (size_t) len - 1);
@@ -274,6 +278,20 @@ int e3d_config_read(const char *path, e3d_params *out, char *mesh_path, int mesh
 		};
 		copy(p.partitionFiles.empty() ? std::string() : p.partitionFiles[0], mesh_path, mesh_path_len);
 		copy(p.outputDir, out_dir, out_dir_len);
+		return E3D_OK;
+	});
+}
+
+int e3d_config_partition(const char *path, int index, char *buf, int len, int *n_part) {
+	if (!path) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		e3d::Params p = e3d::readConfig(path);
+		if (n_part) *n_part = (int) p.partitionFiles.size();
+		if (buf && len > 0) {
+			if (index < 0 || index >= (int) p.partitionFiles.size()) return fail(E3D_ERR_ARG, "partition index out of range");
+			std::strncpy(buf, p.partitionFiles[index].c_str(), (size_t) len - 1);
+			buf[len - 1] = 0;
+		}
 		return E3D_OK;
 	});
 }
@@ -292,8 +310,94 @@ int e3d_mesh_read(const char *path, int require_par, e3d_mesh **out) {
 	});
 }
 
+int e3d_mesh_read_partitions(int n_part, const char *const *paths, e3d_mesh **out) {
+	if (n_part < 1 || !paths || !out) return fail(E3D_ERR_ARG, "null argument or no partition");
+	for (int r = 0; r < n_part; r++)
+		if (!paths[r]) return fail(E3D_ERR_ARG, "null partition path");
+	return guarded([&] {
+		auto m = std::make_unique<e3d_mesh>();
+		for (int r = 0; r < n_part; r++) {
+			try {
+				m->parts.push_back(e3d::readSu2(paths[r], e3d::ParBlock::Partition));
+			} catch (const std::exception &e) {
+				return fail(E3D_ERR_IO, e.what());
+			}
+		}
+		// until preprocessing, the merged copy holds the raw node and element counts
+		for (auto &pf : m->parts) {
+			const int node0 = m->file.nNode();
+			m->file.xyz.insert(m->file.xyz.end(), pf.xyz.begin(), pf.xyz.end());
+			if (m->file.elemStart.empty()) m->file.elemStart.push_back(0);
+			for (int e = 0; e < pf.nElem(); e++) {
+				m->file.elemVtk.push_back(pf.elemVtk[e]);
+				for (int j = pf.elemStart[e]; j < pf.elemStart[e + 1]; j++) m->file.elemNodes.push_back(node0 + pf.elemNodes[j]);
+				m->file.elemStart.push_back((int) m->file.elemNodes.size());
+			}
+		}
+		*out = m.release();
+		return E3D_OK;
+	});
+}
+
+int e3d_mesh_partition_count(const e3d_mesh *m, int *n) {
+	if (!m || !n) return fail(E3D_ERR_ARG, "null argument");
+	*n = m->parts.empty() ? 1 : (int) m->parts.size();
+	return E3D_OK;
+}
+
+int e3d_mesh_partition_info(const e3d_mesh *m, int r, e3d_partition_info *out) {
+	if (!m || !out) return fail(E3D_ERR_ARG, "null argument");
+	if (!m->preprocessed) return fail(E3D_ERR_ARG, "mesh is not preprocessed (call e3d_mesh_preprocess)");
+	const int n = m->parts.empty() ? 1 : (int) m->parts.size();
+	if (r < 0 || r >= n) return fail(E3D_ERR_ARG, "partition index out of range");
+	e3d::PartRange R;
+	if (m->parts.empty()) {
+		R.nElem = m->sm.nElem, R.nNode = m->file.nNode(), R.nFaceInt = m->sm.nFaceInt, R.nGhost = m->sm.nGhost;
+		R.nWall = (int) m->sm.wallGhost.size();
+	} else {
+		R = m->ranges[r];
+	}
+	*out = {R.elem0, R.nElem, R.node0, R.nNode, R.faceInt0, R.nFaceInt, R.ghost0, R.nGhost, R.wall0, R.nWall};
+	return E3D_OK;
+}
+
+int e3d_mesh_partition_assign(const e3d_mesh *m, int n_part, int *elem2part) {
+	if (!m || !elem2part) return fail(E3D_ERR_ARG, "null argument");
+	if (!m->preprocessed) return fail(E3D_ERR_ARG, "mesh is not preprocessed (call e3d_mesh_preprocess)");
+	if (!m->parts.empty()) return fail(E3D_ERR_ARG, "the mesh is already partitioned");
+	return guarded([&] {
+		std::vector<int> p = e3d::partitionRcb(m->sm, n_part);
+		std::copy(p.begin(), p.end(), elem2part);
+		return E3D_OK;
+	});
+}
+
+int e3d_mesh_partition_write(const e3d_mesh *m, int n_part, const int *elem2part, const char *const *paths,
+                             int precision) {
+	if (!m || !paths || n_part < 1) return fail(E3D_ERR_ARG, "null argument");
+	for (int r = 0; r < n_part; r++)
+		if (!paths[r]) return fail(E3D_ERR_ARG, "null partition path");
+	if (!m->preprocessed) return fail(E3D_ERR_ARG, "mesh is not preprocessed (call e3d_mesh_preprocess)");
+	if (!m->parts.empty()) return fail(E3D_ERR_ARG, "the mesh is already partitioned");
+	return guarded([&] {
+		std::vector<int> assign;
+		if (elem2part) assign.assign(elem2part, elem2part + m->sm.nElem);
+		else assign = e3d::partitionRcb(m->sm, n_part);
+		std::vector<e3d::MeshFile> files = e3d::splitMesh(m->file, m->sm, assign, n_part);
+		for (int r = 0; r < n_part; r++) {
+			try {
+				e3d::writeSu2(files[r], paths[r], precision, true);
+			} catch (const std::exception &e) {
+				return fail(E3D_ERR_IO, e.what());
+			}
+		}
+		return E3D_OK;
+	});
+}
+
 int e3d_mesh_write(const e3d_mesh *m, const char *path, int precision, int par_block) {
 	if (!m || !path) return fail(E3D_ERR_ARG, "null argument");
+	if (!m->parts.empty()) return fail(E3D_ERR_ARG, "a mesh of several partitions is written by the partitioner");
 	return guarded([&] {
 		e3d::writeSu2(m->file, path, precision, par_block != 0);
 		return E3D_OK;
@@ -393,7 +497,13 @@ int e3d_mesh_preprocess(e3d_mesh *m, double cref) {
 	if (!m) return fail(E3D_ERR_ARG, "null mesh");
 	if (!(cref > 0)) return fail(E3D_ERR_ARG, "CREF must be positive");
 	return guarded([&] {
-		m->sm = e3d::preprocess(m->file, cref);
+		if (m->parts.empty()) {
+			m->sm = e3d::preprocess(m->file, cref);
+		} else {
+			std::vector<e3d::SolverMesh> sms;
+			for (auto &pf : m->parts) sms.push_back(e3d::preprocess(pf, cref));
+			m->sm = e3d::mergePartitions(m->parts, sms, m->ranges, m->file);
+		}
 		m->preprocessed = true;
 		return E3D_OK;
 	});
@@ -433,6 +543,9 @@ int e3d_mesh_get_i32(const e3d_mesh *m, const char *name, int *out, long long ca
 	else if (s == "symCell") v = &sm.symCell;
 	else if (s == "symFace") v = &sm.symFace;
 	else if (s == "ghostKind") v = &sm.ghostKind;
+	else if (s == "mpiGhost") v = &sm.mpiGhost;
+	else if (s == "mpiCell") v = &sm.mpiCell;
+	else if (s == "partStart") v = &sm.partStart;
 	else if (s == "elemVtk") v = &m->file.elemVtk;
 	else if (s == "elemStart") v = &m->file.elemStart;
 	else if (s == "elemNodes") v = &m->file.elemNodes;
@@ -536,6 +649,15 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		c->plan = {P.nBlocks, P.maxSlots, P.csrStride, c->blockSlot.p, c->se0.p, c->se1.p, c->skind.p,
 		           c->snx.p, c->sny.p, c->snz.p, c->sS.p, c->csr.p, c->spack.p, c->blockHalo.p, c->haloCell.p,
 		           P.maxHalo};
+		// partitions: wall ranges, interface ghosts
+		c->nPart = sm.partStart.size() > 1 ? (int) sm.partStart.size() - 1 : 1;
+		if (!m->ranges.empty()) {
+			for (auto &R : m->ranges) c->wall0.push_back(R.wall0), c->nWallPart.push_back(R.nWall);
+		} else {
+			c->wall0 = {0}, c->nWallPart = {(int) P.wallCell.size()};
+		}
+		c->mpiGhost = sm.mpiGhost, c->mpiCell = sm.mpiCell;
+		c->aeroOut.alloc(6 * (size_t) c->nPart);
 		// wall list in the reference's order for the force integration
 		c->nWall = (int) P.wallCell.size();
 		if (c->nWall) {
@@ -573,9 +695,14 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 			c->nface.alloc(nE), c->nface.upload(nf.data(), nE, s);
 			for (auto *b : {&c->R0, &c->R1, &c->R2}) b->alloc(5 * nE);
 			c->blockMax.alloc(std::max(1, e3d::stageBlocks(c->nElem)));
-			c->smoothFlag.alloc(2);
-			int zero[2] = {0, 0};
-			c->smoothFlag.upload(zero, 2, s);
+			std::vector<int> zero(c->nPart + 1, 0);
+			c->smoothFlag.alloc(zero.size());
+			c->smoothFlag.upload(zero.data(), zero.size(), s);
+			if (c->nPart > 1) {
+				// partitions keep their reference ranges in device order (the cell order groups them)
+				c->cellMax.alloc(nE);
+				c->partStart.alloc(sm.partStart.size()), c->partStart.upload(sm.partStart.data(), sm.partStart.size(), s);
+			}
 			c->sync();
 		}
 		c->fs.alloc(5 * nF);
@@ -631,8 +758,13 @@ int e3d_get_state(e3d_ctx *c, double *const fields[8]) {
 		hipCheck(hipMemcpyAsync(h.data(), c->st[c->cur].p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream),
 		         "hipMemcpyAsync D2H");
 		c->sync();
-		for (int k = 0; k < 8; k++)
-			if (fields[k]) toReference(c, h, k, fields[k]);
+		for (int k = 0; k < 8; k++) {
+			if (!fields[k]) continue;
+			toReference(c, h, k, fields[k]);
+			// an interface ghost holds the copy of its own cell that the last update left in it
+			// (FlowField.cpp:129-138, 159-169)
+			for (size_t i = 0; i < c->mpiGhost.size(); i++) fields[k][c->mpiGhost[i]] = fields[k][c->mpiCell[i]];
+		}
 		return E3D_OK;
 	});
 }
@@ -740,20 +872,36 @@ int e3d_aero(e3d_ctx *c, double cf[3], double cm[3]) {
 	if (!c || !cf || !cm) return fail(E3D_ERR_ARG, "null argument");
 	return guarded([&] {
 		hipCheck(hipSetDevice(c->device), "hipSetDevice");
-		double h[6] = {0, 0, 0, 0, 0, 0};
-		if (c->nWall) {
-			e3d::launchAero(c->nWall, c->wallCell.p, c->wN[0].p, c->wN[1].p, c->wN[2].p, c->wC[0].p, c->wC[1].p, c->wC[2].p,
-			                c->state(), c->cpFactor, c->fr.p_inf, c->prm.cref, c->prm.sref, c->scalars.p, c->stream);
+		// AeroCoefficients::Update per partition (AeroCoefficients.cpp:38-52), then the partitions'
+		// force and moment vectors summed in rank order (MPIHandler::UpdateAerodynamicCoefficients,
+		// MPIHandler.cpp:268-313)
+		std::vector<double> h(6 * (size_t) c->nPart, 0.0);
+		bool any = false;
+		for (int r = 0; r < c->nPart; r++) {
+			if (!c->nWallPart[r]) continue;
+			const int w0 = c->wall0[r];
+			e3d::launchAero(c->nWallPart[r], c->wallCell.p + w0, c->wN[0].p + w0, c->wN[1].p + w0, c->wN[2].p + w0,
+			                c->wC[0].p + w0, c->wC[1].p + w0, c->wC[2].p + w0, c->state(), c->cpFactor, c->fr.p_inf,
+			                c->prm.cref, c->prm.sref, c->aeroOut.p + 6 * (size_t) r, c->stream);
 			c->check();
-			hipCheck(hipMemcpyAsync(h, c->scalars.p, sizeof(h), hipMemcpyDeviceToHost, c->stream), "D2H");
+			any = true;
+		}
+		if (any) {
+			hipCheck(hipMemcpyAsync(h.data(), c->aeroOut.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream), "D2H");
 			c->sync();
 		}
-		// AeroCoefficients.cpp:49-51: lever = ref - CP (CP never divided), M = F x lever / Cref
-		double lever[3] = {c->prm.ref_point[0] - h[3], c->prm.ref_point[1] - h[4], c->prm.ref_point[2] - h[5]};
-		double M[3] = {h[1] * lever[2] - h[2] * lever[1], h[2] * lever[0] - h[0] * lever[2], h[0] * lever[1] - h[1] * lever[0]};
-		for (int k = 0; k < 3; k++) {
-			cf[k] = h[k];
-			cm[k] = M[k] / c->prm.cref;
+		for (int r = 0; r < c->nPart; r++)
+			if (!c->nWallPart[r]) std::fill(h.begin() + 6 * r, h.begin() + 6 * r + 6, 0.0); // no wall face: zero sums
+		for (int r = 0; r < c->nPart; r++) {
+			const double *q = &h[6 * (size_t) r];
+			// AeroCoefficients.cpp:49-51: lever = ref - CP (CP never divided), M = F x lever / Cref
+			double lever[3] = {c->prm.ref_point[0] - q[3], c->prm.ref_point[1] - q[4], c->prm.ref_point[2] - q[5]};
+			double M[3] = {q[1] * lever[2] - q[2] * lever[1], q[2] * lever[0] - q[0] * lever[2], q[0] * lever[1] - q[1] * lever[0]};
+			for (int k = 0; k < 3; k++) {
+				const double mk = M[k] / c->prm.cref;
+				cf[k] = r ? cf[k] + q[k] : q[k];
+				cm[k] = r ? cm[k] + mk : mk;
+			}
 		}
 		return E3D_OK;
 	});

@@ -56,7 +56,7 @@ struct DevMesh {
 	int nElem, nFace, nFaceInt, nGhost;
 	DevFaces f;
 	DevCells c;
-	const int8_t *gkind; // nGhost
+	const int8_t *gkind; // nGhost: 0 farfield, 1 wall, 2 symmetry, 3 partition interface
 };
 
 // block plan of the fused stage kernel
@@ -89,8 +89,10 @@ void launchStage(const DevMesh &m, DevState s, const DevFree &q, const double *f
                  double alpha, int firstStage, int integ, double *rmsPartial, double *Rout, hipStream_t st);
 // residual smoothing R0 -> R2 (R1 scratch, blockMax one double per 256 cells, flag[2] device ints:
 // flag[1] is sticky NaN and must start at 0)
+// flag: [nPart] decisions then the NaN flag; with nPart > 1, partStart (nPart + 1, device order)
+// gives each partition's cells and cellMax (nElem) holds the per-cell changes
 void launchSmooth(const double *R0, double *R1, double *R2, const int *nbr, const uint8_t *nface, int stride, int nElem,
-                  double *blockMax, int *flag, hipStream_t st);
+                  double *blockMax, int *flag, const int *partStart, int nPart, double *cellMax, hipStream_t st);
 int stageBlocks(int nElem);
 // launch choices of the fused stage (they change speed only, never results)
 struct StageTuning {

@@ -19,13 +19,22 @@ namespace e3d {
 enum : int { VTK_TRI = 5, VTK_QUAD = 9, VTK_TET = 10, VTK_HEX = 12, VTK_WEDGE = 13, VTK_PYRAMID = 14 };
 
 // Boundary kind of a ghost cell. The numbering is this library's own.
-enum BoundaryKind : int { BND_FARFIELD = 0, BND_WALL = 1, BND_SYM = 2, BND_OTHER = 3 };
+enum BoundaryKind : int { BND_FARFIELD = 0, BND_WALL = 1, BND_SYM = 2, BND_OTHER = 3, BND_MPI = 4 };
 
 struct Marker {
 	std::string tag;
 	std::vector<int> vtk;       // per boundary element
 	std::vector<int> nodeStart; // CSR into nodes, size n+1
 	std::vector<int> nodes;
+};
+
+// One INTERNAL_MARKER block of a .par file: the faces this partition shares with partition
+// `partner` (MeshPartition.cpp:39-112). Element i is (thisElem[i], otherElem[i]), local ids in the
+// two partitions, and the face's VTK id and nodes (local to this partition).
+struct InterfaceMarker {
+	int partner = 0;
+	std::vector<int> thisElem, otherElem, vtk;
+	std::vector<int> nodeStart{0}, nodes;
 };
 
 // The mesh as a file holds it (SU2 / .par, one partition).
@@ -35,13 +44,19 @@ struct MeshFile {
 	std::vector<int> elemStart;    // nElem + 1
 	std::vector<int> elemNodes;
 	std::vector<Marker> markers;
-	int nInternalMarker = 0;       // .par NINTERNAL_MARKER (must be 0 here: one partition)
+	int nInternalMarker = 0;       // .par NINTERNAL_MARKER
+	std::vector<InterfaceMarker> interfaces; // read with ParBlock::Partition
 	long nElemDeclared = 0;        // NELEM of the file, unknown VTK ids included
 	int nNode() const { return (int) xyz.size() / 3; }
 	int nElem() const { return (int) elemVtk.size(); }
 	int nBoundaryElems() const {
 		int n = 0;
 		for (auto &m : markers) n += (int) m.vtk.size();
+		return n;
+	}
+	int nInterfaceElems() const {
+		int n = 0;
+		for (auto &m : interfaces) n += (int) m.thisElem.size();
 		return n;
 	}
 };
@@ -65,6 +80,13 @@ struct SolverMesh {
 	std::vector<int> farGhost, farCell, farFace;
 	std::vector<int> symGhost, symCell, symFace;
 	std::vector<int> ghostKind;         // nGhost, BoundaryKind of ghost nElem + k
+
+	// partition interfaces (Mesh::updateMPIGhostCells, Mesh.hpp:471-584), in the reference's order
+	std::vector<int> mpiGhost, mpiCell;           // MPIGhostCellIDs, MPIadjacentToGhostCellIDs
+	std::vector<std::vector<int>> interfaceGhost; // per interface marker and element: its ghost
+	// cells [partStart[r], partStart[r + 1]) belong to partition r of a composite mesh (empty for
+	// a mesh read on its own)
+	std::vector<int> partStart;
 
 	// metrics (FP64), non-dimensionalised by CREF
 	std::vector<double> faceNormal;     // 3 * nFace, area-weighted, oriented elem0 -> elem1
@@ -113,10 +135,35 @@ struct Params {
 };
 
 // ---- mesh_io.cpp
-// parBlock: the reader's handling of the .par NINTERNAL_MARKER block
-enum class ParBlock { Ignore, Optional, Required };
+// parBlock: the reader's handling of the .par NINTERNAL_MARKER block. Optional and Required
+// accept only NINTERNAL_MARKER= 0 (a mesh solved on its own); Partition requires the block and
+// reads its interface markers (one partition of several).
+enum class ParBlock { Ignore, Optional, Required, Partition };
 MeshFile readSu2(const std::string &path, ParBlock parBlock);
 void writeSu2(const MeshFile &m, const std::string &path, int precision, bool parBlock);
+
+// ---- partitions.cpp
+// Where partition r of a composite mesh sits in it.
+struct PartRange {
+	int elem0 = 0, nElem = 0;       // interior cells
+	int node0 = 0, nNode = 0;       // nodes (composite node ids = node0 + local id)
+	int faceInt0 = 0, nFaceInt = 0; // interior faces
+	int ghost0 = 0, nGhost = 0;     // ghosts; boundary face nFaceInt + ghost0 + k of the composite
+	int wall0 = 0, nWall = 0;       // entries of the composite wall list
+};
+// Several partitions of one mesh (each preprocessed on its own, in its reference numbering) as one
+// resident problem. Cells, nodes and ghosts are concatenated partition by partition; faces are
+// [every partition's interior faces][every partition's boundary faces], so each cell still sums
+// its faces in its partition's ascending face order. An interface face keeps its ghost, and its
+// elem1 becomes the cell of the other partition whose state the reference's ghost exchange
+// delivers to that ghost (MPIHandler.cpp:16-267).
+SolverMesh mergePartitions(const std::vector<MeshFile> &files, const std::vector<SolverMesh> &parts,
+                           std::vector<PartRange> &ranges, MeshFile &merged);
+// Element-to-partition assignment by recursive coordinate bisection of the cell centroids.
+std::vector<int> partitionRcb(const SolverMesh &sm, int nPart);
+// The partition files E3D_PRE writes for an assignment (Partition.cpp:123-460).
+std::vector<MeshFile> splitMesh(const MeshFile &global, const SolverMesh &sm, const std::vector<int> &elem2part,
+                                int nPart);
 
 // ---- config.cpp
 Params readConfig(const std::string &path);

@@ -268,7 +268,7 @@ __device__ __forceinline__ Ghost farfieldGhost(const DevState &in, int g, double
 // EulerSolver::updateBC (EulerSolver.cpp:201-247), one thread per ghost, in place
 __global__ void __launch_bounds__(kBlock) kBc(DevMesh m, DevState s, DevFree q, int machDerived) {
 	int k = blockIdx.x * kBlock + threadIdx.x;
-	if (k >= m.nGhost) return;
+	if (k >= m.nGhost || m.gkind[k] == 3) return; // an interface face reads the other partition's cell
 	int f = m.nFaceInt + k;
 	int g = m.nElem + k;
 	int i = m.f.e0[f];
@@ -497,9 +497,10 @@ __device__ __forceinline__ void smoothCell(const double *R0, const double *last,
 	for (int k = 0; k < 5; k++) out[k] = (R0[k * n + e] + sum[k]) / den;
 }
 
-// first sweep: R1 and the largest change per block (NaN when a change is NaN)
+// first sweep: R1 and the largest change per block (NaN when a change is NaN), or per cell into
+// cellMax when the cells belong to several partitions (each decides on its own)
 __global__ void __launch_bounds__(kBlock) kSmooth1(const double *R0, double *R1, const int *nbr, const uint8_t *nface,
-                                                   int stride, int nElem, double *blockMax) {
+                                                   int stride, int nElem, double *blockMax, double *cellMax) {
 	__shared__ double lds[kBlock];
 	const int e = blockIdx.x * kBlock + threadIdx.x;
 	const size_t n = (size_t) nElem;
@@ -513,7 +514,9 @@ __global__ void __launch_bounds__(kBlock) kSmooth1(const double *R0, double *R1,
 			const double c = fabs(R0[k * n + e] - o[k]);
 			d = (c > d || c != c) ? c : d;
 		}
+		if (cellMax) cellMax[e] = d;
 	}
+	if (cellMax) return;
 	lds[threadIdx.x] = d;
 	__syncthreads();
 	for (int off = kBlock / 2; off > 0; off >>= 1) {
@@ -524,6 +527,33 @@ __global__ void __launch_bounds__(kBlock) kSmooth1(const double *R0, double *R1,
 		__syncthreads();
 	}
 	if (threadIdx.x == 0) blockMax[blockIdx.x] = lds[0];
+}
+
+// per-partition decisions (one workgroup per partition, over its contiguous cells): flag[r] = 1
+// when partition r's first sweep changed something by more than 1e-16; flag[nPart] = 1 on a NaN
+__global__ void __launch_bounds__(kBlock) kSmoothDecideParts(const double *cellMax, const int *partStart, int nPart,
+                                                             int *flag) {
+	__shared__ double lds[kBlock];
+	const int r = blockIdx.x;
+	double d = 0.0;
+	for (int i = partStart[r] + (int) threadIdx.x; i < partStart[r + 1]; i += kBlock) {
+		const double c = cellMax[i];
+		d = (c > d || c != c) ? c : d;
+	}
+	lds[threadIdx.x] = d;
+	__syncthreads();
+	for (int off = kBlock / 2; off > 0; off >>= 1) {
+		if ((int) threadIdx.x < off) {
+			const double a = lds[threadIdx.x], b = lds[threadIdx.x + off];
+			lds[threadIdx.x] = (b > a || b != b) ? b : a;
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		const double err = lds[0];
+		flag[r] = err > 1e-16 ? 1 : 0;
+		if (err != err) atomicOr(flag + nPart, 1);
+	}
 }
 
 // flag[0] = 1 when the first sweep changed something by more than 1e-16, flag[1] = 1 on a NaN change
@@ -550,13 +580,21 @@ __global__ void __launch_bounds__(kBlock) kSmoothDecide(const double *blockMax, 
 	}
 }
 
-// second sweep from R1 into R2 when flag[0], else R2 = R1
+// second sweep from R1 into R2 when the cell's partition decided so (flag[partition]), else R2 = R1
 __global__ void __launch_bounds__(kBlock) kSmooth2(const double *R0, const double *R1, double *R2, const int *nbr,
-                                                   const uint8_t *nface, int stride, int nElem, const int *flag) {
+                                                   const uint8_t *nface, int stride, int nElem, const int *flag,
+                                                   const int *partStart, int nPart) {
 	const int e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= nElem) return;
 	const size_t n = (size_t) nElem;
-	if (flag[0]) {
+	int r = 0;
+	for (int lo = 0, hi = nPart - 1; lo < hi;) { // last r with partStart[r] <= e
+		const int mid = (lo + hi + 1) >> 1;
+		if (partStart[mid] <= e) lo = mid;
+		else hi = mid - 1;
+		r = lo;
+	}
+	if (flag[r]) {
 		double o[5];
 		smoothCell(R0, R1, nbr, nface, stride, n, e, o);
 #pragma unroll
@@ -875,12 +913,19 @@ void launchStage(const DevMesh &m, DevState s, const DevFree &q, const double *f
 }
 
 void launchSmooth(const double *R0, double *R1, double *R2, const int *nbr, const uint8_t *nface, int stride, int nElem,
-                  double *blockMax, int *flag, hipStream_t st) {
+                  double *blockMax, int *flag, const int *partStart, int nPart, double *cellMax, hipStream_t st) {
 	if (nElem == 0) return;
 	const int nb = blocksFor(nElem);
-	hipLaunchKernelGGL(kSmooth1, dim3(nb), dim3(kBlock), 0, st, R0, R1, nbr, nface, stride, nElem, blockMax);
-	hipLaunchKernelGGL(kSmoothDecide, dim3(1), dim3(kBlock), 0, st, blockMax, nb, flag);
-	hipLaunchKernelGGL(kSmooth2, dim3(nb), dim3(kBlock), 0, st, R0, R1, R2, nbr, nface, stride, nElem, flag);
+	if (nPart > 1) {
+		hipLaunchKernelGGL(kSmooth1, dim3(nb), dim3(kBlock), 0, st, R0, R1, nbr, nface, stride, nElem, blockMax, cellMax);
+		hipLaunchKernelGGL(kSmoothDecideParts, dim3(nPart), dim3(kBlock), 0, st, cellMax, partStart, nPart, flag);
+	} else {
+		hipLaunchKernelGGL(kSmooth1, dim3(nb), dim3(kBlock), 0, st, R0, R1, nbr, nface, stride, nElem, blockMax,
+		                   (double *) nullptr);
+		hipLaunchKernelGGL(kSmoothDecide, dim3(1), dim3(kBlock), 0, st, blockMax, nb, flag);
+	}
+	hipLaunchKernelGGL(kSmooth2, dim3(nb), dim3(kBlock), 0, st, R0, R1, R2, nbr, nface, stride, nElem, flag, partStart,
+	                   nPart);
 }
 
 void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const DevFree &q, int scheme,

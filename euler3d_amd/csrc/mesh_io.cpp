@@ -3,8 +3,7 @@
 // Reader semantics follow src/parser/SU2MeshParser.cpp:32-240 (NDIME must be 3; NELEM, NPOIN and
 // NMARK blocks, each keyword searched forward from the previous block) and
 // src/parser/MeshPartition.cpp:39-112 (the .par NINTERNAL_MARKER block, which is required for a
-// partition file). Only one partition is supported: a non-zero NINTERNAL_MARKER is an error,
-// because the whole mesh is resident on one device.
+// partition file, and its interface markers when several partitions are read together).
 //
 // Writer layout follows src/partition/SU2Writer.cpp:29-123 (headers, setw(3) vtk ids,
 // setw(10) node ids and element index, setw(20) scientific coordinates at the requested
@@ -177,9 +176,35 @@ MeshFile readSu2(const std::string &path, ParBlock parBlock) {
 	Cursor c2 = c;
 	if (c2.findLine("NINTERNAL_MARKER=", b, e)) {
 		m.nInternalMarker = (int) parseLongAfter(b, e, 17);
-		if (m.nInternalMarker != 0)
-			throw std::runtime_error("NINTERNAL_MARKER > 0: multi-partition .par input is not supported (one resident mesh per device)");
-	} else if (parBlock == ParBlock::Required) {
+		if (parBlock == ParBlock::Partition) {
+			// MeshPartition::parseMpiBoundaryElement (MeshPartition.cpp:39-112): INTERNAL_MARKER=,
+			// NELEM_MPI=, then "this other vtk nodes..." per element; a face VTK id it does not
+			// know carries no nodes
+			for (int im = 0; im < m.nInternalMarker; im++) {
+				InterfaceMarker mk;
+				if (!c2.line(b, e)) throw std::runtime_error("missing INTERNAL_MARKER");
+				mk.partner = (int) parseLongAfter(b, e, 16);
+				if (!c2.line(b, e)) throw std::runtime_error("missing NELEM_MPI");
+				long ne = parseLongAfter(b, e, 10);
+				for (long i = 0; i < ne; i++) {
+					if (!c2.line(b, e)) throw std::runtime_error("unexpected end of file in NELEM_MPI block");
+					int n = lineInts(b, e, vals, 16);
+					if (n < 3) throw std::runtime_error("malformed interface element line");
+					int nn = nodesOfSurface((int) vals[2]);
+					if (nn < 0) nn = 0;
+					if (n < 3 + nn) throw std::runtime_error("interface element line has too few nodes");
+					mk.thisElem.push_back((int) vals[0]);
+					mk.otherElem.push_back((int) vals[1]);
+					mk.vtk.push_back((int) vals[2]);
+					for (int k = 0; k < nn; k++) mk.nodes.push_back((int) vals[3 + k]);
+					mk.nodeStart.push_back((int) mk.nodes.size());
+				}
+				m.interfaces.push_back(std::move(mk));
+			}
+		} else if (m.nInternalMarker != 0) {
+			throw std::runtime_error("NINTERNAL_MARKER > 0: this partition has neighbours; read all partitions together");
+		}
+	} else if (parBlock == ParBlock::Required || parBlock == ParBlock::Partition) {
 		throw std::runtime_error("MPI Boundary Elements keyword \"NINTERNAL_MARKER=\" not found !");
 	}
 	return m;
@@ -218,8 +243,19 @@ void writeSu2(const MeshFile &m, const std::string &path, int precision, bool pa
 		}
 	}
 	if (parBlock) {
+		// SU2Writer::WriteInternalMarker (SU2Writer.cpp:98-121): one block per neighbour partition,
+		// each element as setw(10) fields this, other, vtk, face nodes
 		std::fprintf(f, "%%\n%% Internal Boundary elements\n%%\n");
-		std::fprintf(f, "NINTERNAL_MARKER= 0\n");
+		std::fprintf(f, "NINTERNAL_MARKER= %d\n", (int) m.interfaces.size());
+		for (auto &mk : m.interfaces) {
+			std::fprintf(f, "INTERNAL_MARKER= %d\n", mk.partner);
+			std::fprintf(f, "NELEM_MPI= %d\n", (int) mk.thisElem.size());
+			for (size_t i = 0; i < mk.thisElem.size(); i++) {
+				std::fprintf(f, "%10d%10d%10d", mk.thisElem[i], mk.otherElem[i], mk.vtk[i]);
+				for (int k = mk.nodeStart[i]; k < mk.nodeStart[i + 1]; k++) std::fprintf(f, "%10d", mk.nodes[k]);
+				std::fprintf(f, "\n");
+			}
+		}
 	}
 	std::fclose(f);
 }

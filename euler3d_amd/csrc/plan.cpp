@@ -47,9 +47,17 @@ std::vector<int> mortonOrder(const SolverMesh &sm) {
 		}
 		key[e] = spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2;
 	}
+	// the cells of a composite mesh stay grouped by partition (Morton order inside each), so that
+	// per-partition reductions run over contiguous ranges
+	std::vector<int> part(n, 0);
+	for (size_t r = 0; r + 1 < sm.partStart.size(); r++)
+		for (int e = sm.partStart[r]; e < sm.partStart[r + 1]; e++) part[e] = (int) r;
 	std::vector<int> order(n);
 	std::iota(order.begin(), order.end(), 0);
-	std::sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b] || (key[a] == key[b] && a < b); });
+	std::sort(order.begin(), order.end(), [&](int a, int b) {
+		if (part[a] != part[b]) return part[a] < part[b];
+		return key[a] < key[b] || (key[a] == key[b] && a < b);
+	});
 	return order;
 }
 
@@ -58,6 +66,7 @@ int8_t slotKind(const SolverMesh &sm, int f) {
 	switch (sm.ghostKind[f - sm.nFaceInt]) {
 		case BND_WALL: return 1;
 		case BND_SYM: return 2;
+		case BND_MPI: return 0; // the other partition's cell is elem1: a one-sided interior face
 		default: return 3;
 	}
 }
@@ -94,7 +103,7 @@ HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock) {
 	}
 	P.gkind.resize(sm.nGhost);
 	for (int k = 0; k < sm.nGhost; k++)
-		P.gkind[k] = (int8_t) (sm.ghostKind[k] == BND_WALL ? 1 : sm.ghostKind[k] == BND_SYM ? 2 : 0);
+		P.gkind[k] = (int8_t) (sm.ghostKind[k] == BND_WALL ? 1 : sm.ghostKind[k] == BND_SYM ? 2 : sm.ghostKind[k] == BND_MPI ? 3 : 0);
 
 	// cells in device order: static time-step sums (local-face order, TimeIntegration.cpp:20-26)
 	// and the face list in ascending reference face id

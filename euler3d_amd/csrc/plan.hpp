@@ -18,7 +18,7 @@ struct HostPlan {
 	// per-face arrays in reference face order, cell ids in device numbering
 	std::vector<int> fe0, fe1;
 	std::vector<double> fnx, fny, fnz, fS;
-	std::vector<int8_t> gkind; // per ghost: 0 farfield, 1 wall, 2 symmetry
+	std::vector<int8_t> gkind; // per ghost: 0 farfield, 1 wall, 2 symmetry, 3 partition interface
 
 	// per-cell arrays in device order
 	std::vector<double> vol, sx, sy, sz;

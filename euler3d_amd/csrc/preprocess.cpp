@@ -207,9 +207,48 @@ SolverMesh preprocess(const MeshFile &m, double cref) {
 	s.face2nodeStart = std::move(f2nStart);
 	s.face2node = std::move(f2n);
 
-	if (m.nBoundaryElems() != s.nGhost)
-		throw std::runtime_error("boundary marker elements (" + std::to_string(m.nBoundaryElems()) +
+	if (m.nBoundaryElems() + m.nInterfaceElems() != s.nGhost)
+		throw std::runtime_error("boundary marker and interface elements (" +
+		                         std::to_string(m.nBoundaryElems() + m.nInterfaceElems()) +
 		                         ") do not match the mesh's boundary faces (" + std::to_string(s.nGhost) + ")");
+
+	// ---- partition interfaces (Mesh::updateMPIGhostCells, Mesh.hpp:471-584): an interface
+	// element's ghost is its cell's only ghost neighbour, or, when the cell has several, the one
+	// whose face has the element's nodes
+	std::vector<uint8_t> inMpi(s.nGhost, 0);
+	s.interfaceGhost.resize(m.interfaces.size());
+	for (size_t im = 0; im < m.interfaces.size(); im++) {
+		const InterfaceMarker &mk = m.interfaces[im];
+		s.interfaceGhost[im].assign(mk.thisElem.size(), -1);
+		for (size_t i = 0; i < mk.thisElem.size(); i++) {
+			const int e = mk.thisElem[i];
+			if (e < 0 || e >= nElem) throw std::runtime_error("interface element refers to cell " + std::to_string(e) + " outside the partition");
+			int cand[6], nc = 0;
+			for (int j = s.element2faceStart[e]; j < s.element2faceStart[e + 1]; j++)
+				if (s.element2element[j] >= nElem) cand[nc++] = s.element2element[j];
+			int g = -1;
+			if (nc == 1) {
+				g = cand[0];
+			} else if (nc > 1) {
+				std::vector<int> want(mk.nodes.begin() + mk.nodeStart[i], mk.nodes.begin() + mk.nodeStart[i + 1]);
+				std::sort(want.begin(), want.end());
+				for (int c = 0; c < nc && g < 0; c++) {
+					int f = s.nFaceInt + (cand[c] - nElem);
+					std::vector<int> have(s.face2node.begin() + s.face2nodeStart[f], s.face2node.begin() + s.face2nodeStart[f + 1]);
+					std::sort(have.begin(), have.end());
+					if (have == want) g = cand[c];
+				}
+			}
+			if (g < 0)
+				throw std::runtime_error("interface element " + std::to_string(i) + " toward partition " + std::to_string(mk.partner) +
+				                         " (cell " + std::to_string(e) + ") matches no boundary face");
+			if (inMpi[g - nElem]) throw std::runtime_error("two interface elements resolve to ghost " + std::to_string(g));
+			inMpi[g - nElem] = 1;
+			s.interfaceGhost[im][i] = g;
+			s.mpiGhost.push_back(g);
+			s.mpiCell.push_back(e);
+		}
+	}
 
 	// ---- ghost classification (Mesh.hpp:594-707): marker order, first matching boundary face
 	{
@@ -256,9 +295,11 @@ SolverMesh preprocess(const MeshFile &m, double cref) {
 				throw std::runtime_error("unsupported boundary tag '" + mk.tag +
 				                         "' (the solver knows farfield, wall/airfoil and sym)");
 		}
-		// residual dispatch precedence of EulerSolver.cpp:331-357: wall, then sym, else farfield
+		// residual dispatch precedence of EulerSolver.cpp:331-357: wall, then interface or sym,
+		// else farfield
 		for (int k = 0; k < s.nGhost; k++) {
 			if (inWall[k]) s.ghostKind[k] = BND_WALL;
+			else if (inMpi[k]) s.ghostKind[k] = BND_MPI;
 			else if (inSym[k]) s.ghostKind[k] = BND_SYM;
 			else if (inFar[k]) s.ghostKind[k] = BND_FARFIELD;
 			else throw std::runtime_error("boundary face " + std::to_string(s.nFaceInt + k) + " is on no marker");

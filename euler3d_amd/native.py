@@ -40,6 +40,12 @@ class MeshInfo(C.Structure):
                 ("n_node", "n_elem", "n_face", "n_face_int", "n_ghost", "n_wall", "n_far", "n_sym")]
 
 
+class PartitionInfo(C.Structure):
+    _fields_ = [(k, C.c_int) for k in
+                ("elem0", "n_elem", "node0", "n_node", "face_int0", "n_face_int", "ghost0", "n_ghost", "wall0",
+                 "n_wall")]
+
+
 class BatchOpts(C.Structure):
     _fields_ = [("hbm_budget", C.c_double), ("max_cases", C.c_int), ("wg_per_cu", C.c_int), ("xcds", C.c_int),
                 ("nontemporal", C.c_int), ("use_graph", C.c_int)]
@@ -61,7 +67,13 @@ _SIGS = {
     "e3d_version": (C.c_char_p, []),
     "e3d_params_default": (C.c_int, [C.POINTER(Params)]),
     "e3d_config_read": (C.c_int, [C.c_char_p, C.POINTER(Params), C.c_char_p, C.c_int, C.c_char_p, C.c_int]),
+    "e3d_config_partition": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p, C.c_int, IP]),
     "e3d_mesh_read": (C.c_int, [C.c_char_p, C.c_int, PP]),
+    "e3d_mesh_read_partitions": (C.c_int, [C.c_int, C.POINTER(C.c_char_p), PP]),
+    "e3d_mesh_partition_count": (C.c_int, [P, IP]),
+    "e3d_mesh_partition_info": (C.c_int, [P, C.c_int, C.POINTER(PartitionInfo)]),
+    "e3d_mesh_partition_assign": (C.c_int, [P, C.c_int, IP]),
+    "e3d_mesh_partition_write": (C.c_int, [P, C.c_int, IP, C.POINTER(C.c_char_p), C.c_int]),
     "e3d_mesh_write": (C.c_int, [P, C.c_char_p, C.c_int, C.c_int]),
     "e3d_mesh_gen_box": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_char_p, PP]),
     "e3d_mesh_gen_naca": (C.c_int, [C.c_int, C.c_int, C.c_int] + [C.c_double] * 6 + [PP]),

@@ -23,7 +23,8 @@ STATE_FIELDS = ("rho", "u", "v", "w", "p", "E", "H", "M")
 
 I32_ARRAYS = ("face2element", "face2nodeStart", "face2node", "element2faceStart", "element2face",
               "element2element", "wallGhost", "wallCell", "wallFace", "farGhost", "farCell", "farFace",
-              "symGhost", "symCell", "symFace", "ghostKind", "elemVtk", "elemStart", "elemNodes")
+              "symGhost", "symCell", "symFace", "ghostKind", "elemVtk", "elemStart", "elemNodes", "mpiGhost",
+              "mpiCell", "partStart")
 F64_ARRAYS = ("faceNormals", "faceNormalsUnit", "faceSurfaces", "faceCenters", "cellVolumes",
               "cellCentroids", "xyz")
 
@@ -40,6 +41,14 @@ class Mesh:
     def read(cls, path: str, par: bool = True) -> "Mesh":
         h = C.c_void_p()
         N.check(N.lib().e3d_mesh_read(str(path).encode(), int(par), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def read_partitions(cls, paths) -> "Mesh":
+        """The partition files of one multi-rank run (rank r reads paths[r]) as one mesh."""
+        arr = (C.c_char_p * len(paths))(*[str(p).encode() for p in paths])
+        h = C.c_void_p()
+        N.check(N.lib().e3d_mesh_read_partitions(len(paths), arr, C.byref(h)))
         return cls(h.value)
 
     @classmethod
@@ -73,6 +82,34 @@ class Mesh:
     # ---- operations
     def write(self, path: str, precision: int = 16, par: bool = True) -> None:
         N.check(N.lib().e3d_mesh_write(self._h, str(path).encode(), precision, int(par)))
+
+    def write_partitions(self, paths, elem2part=None, precision: int = 10) -> None:
+        """E3D_PRE: the partition files of this (preprocessed, single) mesh; elem2part None uses
+        recursive coordinate bisection."""
+        arr = (C.c_char_p * len(paths))(*[str(p).encode() for p in paths])
+        ep = None
+        if elem2part is not None:
+            ep = np.ascontiguousarray(elem2part, dtype=np.int32)
+            if ep.shape != (self.info()["n_elem"],):
+                raise ValueError("one partition id per cell is required")
+        N.check(N.lib().e3d_mesh_partition_write(self._h, len(paths), None if ep is None else N.iptr(ep), arr,
+                                                 precision))
+
+    def partition_assign(self, n_part: int) -> np.ndarray:
+        out = np.empty(self.info()["n_elem"], dtype=np.int32)
+        N.check(N.lib().e3d_mesh_partition_assign(self._h, n_part, N.iptr(out)))
+        return out
+
+    def partitions(self) -> list[dict]:
+        """Where each partition sits in the (preprocessed) mesh: cells, nodes, faces, ghosts, walls."""
+        n = C.c_int()
+        N.check(N.lib().e3d_mesh_partition_count(self._h, C.byref(n)))
+        out = []
+        for r in range(n.value):
+            q = N.PartitionInfo()
+            N.check(N.lib().e3d_mesh_partition_info(self._h, r, C.byref(q)))
+            out.append({k: getattr(q, k) for k, _ in N.PartitionInfo._fields_})
+        return out
 
     def preprocess(self, cref: float = 1.0) -> "Mesh":
         N.check(N.lib().e3d_mesh_preprocess(self._h, float(cref)))
@@ -115,10 +152,11 @@ class Mesh:
 
 @dataclass
 class SimConfig:
-    """Parsed .e3d file (SimConfig): solver parameters, the partition file and the output dir."""
+    """Parsed .e3d file (SimConfig): solver parameters, the partition files and the output dir."""
     params: N.Params
     mesh_path: str
     output_dir: str
+    partition_files: tuple = ()
 
     @classmethod
     def read(cls, path: str) -> "SimConfig":
@@ -126,7 +164,14 @@ class SimConfig:
         mesh = C.create_string_buffer(4096)
         out = C.create_string_buffer(4096)
         N.check(N.lib().e3d_config_read(str(path).encode(), C.byref(p), mesh, 4096, out, 4096))
-        return cls(p, mesh.value.decode(), out.value.decode())
+        n = C.c_int()
+        N.check(N.lib().e3d_config_partition(str(path).encode(), 0, None, 0, C.byref(n)))
+        files = []
+        for i in range(n.value):
+            buf = C.create_string_buffer(4096)
+            N.check(N.lib().e3d_config_partition(str(path).encode(), i, buf, 4096, None))
+            files.append(buf.value.decode())
+        return cls(p, mesh.value.decode(), out.value.decode(), tuple(files))
 
 
 class EulerSolver:

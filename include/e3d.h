@@ -19,6 +19,9 @@
  *   SU2MeshParser / MeshPartition src/parser/SU2MeshParser.cpp:12-240,
  *                                 src/parser/MeshPartition.cpp:15-112  -> e3d_mesh_read
  *   SU2Writer                     src/partition/SU2Writer.cpp:17-123   -> e3d_mesh_write
+ *   MeshPartition x N ranks + MPIHandler  MeshPartition.cpp:15-112,
+ *                                 MPIHandler.cpp:16-313               -> e3d_mesh_read_partitions
+ *   Partition::Write (E3D_PRE)    src/partition/Partition.cpp:357-460  -> e3d_mesh_partition_write
  *   Post(config).Write()          src/post/post.cpp:18-133             -> e3d_post_tecplot
  *   Mesh::solveConnectivity + Metrics  include/mesh/Mesh.hpp:36-105,
  *                                 src/mesh/Metrics.cpp:9-56            -> e3d_mesh_preprocess
@@ -85,9 +88,12 @@ const char *e3d_version(void);
 /* ---- configuration (.e3d) ---------------------------------------------------------------- */
 int e3d_params_default(e3d_params *out);
 /* Parses an .e3d file. mesh_path receives the first partition file (PARTITION_FILES pattern with
- * '#' -> 0), out_dir the directory of OUTPUT_FILE. Partition counts other than 1 are rejected. */
+ * '#' -> 0), out_dir the directory of OUTPUT_FILE. */
 int e3d_config_read(const char *path, e3d_params *out, char *mesh_path, int mesh_path_len, char *out_dir,
                     int out_dir_len);
+/* SimConfig::getPartitionedMeshFiles (SimConfig.cpp:52-66): partition file `index` of the .e3d
+ * file into buf (may be NULL) and the partition count into *n_part (may be NULL). */
+int e3d_config_partition(const char *path, int index, char *buf, int len, int *n_part);
 
 /* ---- meshes ------------------------------------------------------------------------------- */
 /* SU2 (.su2) or partition (.par, require_par = 1: the NINTERNAL_MARKER block must be present) */
@@ -95,6 +101,36 @@ int e3d_mesh_read(const char *path, int require_par, e3d_mesh **out);
 /* SU2Writer layout; precision = digits after the point of the %e coordinates; par_block adds
  * "NINTERNAL_MARKER= 0" so the file is a one-partition .par */
 int e3d_mesh_write(const e3d_mesh *m, const char *path, int precision, int par_block);
+
+/* ---- several partitions ------------------------------------------------------------------- */
+/* The n_part partition files one multi-rank run reads, rank r reading paths[r]
+ * (Mesh<MeshPartition>, MeshPartition.cpp:15-112, src/Solver.cpp:37-49), as ONE mesh: after
+ * e3d_mesh_preprocess every partition is numbered as the reference numbers it on its rank, and
+ * the composite's cells, nodes, ghosts and wall faces are the partitions' in rank order. A context
+ * created from it runs the reference's multi-rank solve on one device: interface faces read the
+ * cell that the reference's ghost exchange delivers (MPIHandler.cpp:16-267), the RMS sums all
+ * partitions, force and moment coefficients are summed per partition in rank order
+ * (MPIHandler.cpp:268-313), and residual smoothing stops per partition. */
+int e3d_mesh_read_partitions(int n_part, const char *const *paths, e3d_mesh **out);
+typedef struct e3d_partition_info {
+	int elem0, n_elem;       /* interior cells of the partition in the composite */
+	int node0, n_node;       /* its nodes */
+	int face_int0, n_face_int; /* its interior faces */
+	int ghost0, n_ghost;     /* its ghosts: composite ghost n_elem_total + ghost0 + k, boundary face
+	                          * n_face_int_total + ghost0 + k */
+	int wall0, n_wall;       /* its entries of the wall lists (wallCell, wallFace, ...) */
+} e3d_partition_info;
+/* 1 for a mesh read on its own */
+int e3d_mesh_partition_count(const e3d_mesh *m, int *n);
+int e3d_mesh_partition_info(const e3d_mesh *m, int r, e3d_partition_info *out);
+/* E3D_PRE for n_part > 1 (Partition::Write, Partition.cpp:357-460): writes the partition files of
+ * a preprocessed single mesh, paths[r] for partition r, coordinates with `precision` digits (the
+ * reference writes 10). elem2part gives each cell's partition; NULL uses this library's recursive
+ * coordinate bisection (the reference calls METIS, which its sources do not include). */
+int e3d_mesh_partition_write(const e3d_mesh *m, int n_part, const int *elem2part, const char *const *paths,
+                             int precision);
+/* The recursive-coordinate-bisection assignment e3d_mesh_partition_write uses by default. */
+int e3d_mesh_partition_assign(const e3d_mesh *m, int n_part, int *elem2part);
 /* Post::Post(config) + Post::Write()  src/post/post.cpp:18-133 (the E3D_POST executable,
  * src/postprocessor.cpp:17-40): writes the Tecplot ASCII file OUTPUT_FILE of the .e3d file, one
  * FEBRICK zone per listed partition, with the cell values read from

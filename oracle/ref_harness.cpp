@@ -18,6 +18,7 @@
 //                                        (src/solver/EulerSolver.cpp:201-359, 425-472)
 //   run  <cfg.e3d> <out.txt>            EulerSolver::Run() for MAX_ITER iterations, final state,
 //                                        last residual and coefficients (EulerSolver.cpp:60-167)
+// Under mpiexec -np N (partitioned input) every rank writes <out.txt>.r<rank> for its partition.
 
 // Open the private members of the reference classes to this harness only: the hot-path pieces
 // (updateBC, computeResidual, RungeKutta, _residuals) are private in EulerSolver.hpp.
@@ -155,6 +156,8 @@ void modeMesh(Setup &s, Dump &d) {
 	d.ints("symGhost", m.GetSymmetryGhostCellsIDs());
 	d.ints("symCell", m.GetSymmetryAdjacentGhostCellIDs());
 	d.ints("symFace", m.GetSymmetryAdjacentFaceIDs());
+	d.ints("mpiGhost", m.GetMPIGhostCellsIDs());
+	d.ints("mpiCell", m.getMPIadjacentToGhostCellIDs());
 	auto &mt = *s.metrics;
 	d.vec3("faceNormals", mt.getFaceNormals());
 	d.vec3("faceNormalsUnit", mt.getFaceNormalsUnit());
@@ -210,6 +213,33 @@ void modeFlux(Setup &s, Dump &d, uint64_t seed) {
 	auto &m = s.mesh;
 	auto &mt = *s.metrics;
 	seedState(s, seed);
+	if (s.mpi.getPoolSize() > 1) {
+		// partitioned run: make every interior state one that an update could leave, p and H from
+		// (rho, u, v, w, E) as FlowField::Update forms them (FlowField.cpp:121-122), because the
+		// exchange recomputes them that way on the receiving side (MPIHandler.cpp:184-196)
+		double g = ff.getgamma_ref();
+		for (int i = 0; i < s.nElem(); i++) {
+			double rho = ff.Getrho()[i], u = ff.GetU_Velocity()[i], v = ff.GetV_Velocity()[i], w = ff.GetW_Velocity()[i];
+			double E = ff.GetE()[i];
+			double p = (g - 1) * rho * (E - (std::pow(u, 2) + std::pow(v, 2) + std::pow(w, 2)) / 2.0);
+			ff.setP(i, p);
+			ff.setH(i, E + (p / rho));
+		}
+	}
+	// interface ghosts hold their own cell's copy, as every update leaves them
+	// (FlowField.cpp:129-138); the exchange in updateBC then brings the other partition's cells
+	const auto &mg = m.GetMPIGhostCellsIDs();
+	const auto &mc = m.getMPIadjacentToGhostCellIDs();
+	for (size_t i = 0; i < mg.size(); i++) {
+		ff.setRho(mg[i], ff.Getrho()[mc[i]]);
+		ff.setU(mg[i], ff.GetU_Velocity()[mc[i]]);
+		ff.setV(mg[i], ff.GetV_Velocity()[mc[i]]);
+		ff.setW(mg[i], ff.GetW_Velocity()[mc[i]]);
+		ff.setP(mg[i], ff.GetP()[mc[i]]);
+		ff.setE(mg[i], ff.GetE()[mc[i]]);
+		ff.setH(mg[i], ff.GetH()[mc[i]]);
+		ff.setM(mg[i], ff.GetMach()[mc[i]]);
+	}
 	dumpState(d, ff, "in_");
 
 	// Same preamble as EulerSolver::Run (EulerSolver.cpp:65-78)
@@ -239,6 +269,9 @@ void modeFlux(Setup &s, Dump &d, uint64_t seed) {
 			auto n = mt.getFaceNormalsUnit()[f];
 			r = Solver::ResidualVar(0, n.x * ff.GetP()[e2], n.y * ff.GetP()[e2], n.z * ff.GetP()[e2], 0);
 			kind[f] = 1;
+		} else if (std::binary_search(sv._sortedMPIGhostCellIDs.begin(), sv._sortedMPIGhostCellIDs.end(), e2)) {
+			r = Solver::Roe(ff, m, mt, f);
+			kind[f] = 4;
 		} else if (std::binary_search(sv._sortedSymmetryGhostCellIDs.begin(), sv._sortedSymmetryGhostCellIDs.end(), e2)) {
 			r = Solver::Roe(ff, m, mt, f);
 			kind[f] = 2;
@@ -300,7 +333,10 @@ int main(int argc, char **argv) {
 	std::string mode = argv[1];
 	{
 		Setup s(mpi, argv[2]);
-		Dump d(argv[3]);
+		// several ranks: one dump per rank, <out>.r<rank>
+		std::string out = argv[3];
+		if (mpi.getPoolSize() > 1) out += ".r" + std::to_string(mpi.getRankID());
+		Dump d(out.c_str());
 		if (mode == "mesh") {
 			modeMesh(s, d);
 		} else if (mode == "flux") {

@@ -22,6 +22,15 @@ Cases:
   bump_smooth         bump_small mesh, RK5 with RESIDUAL_SMOOTHING= 1, 10 iterations
   wing_euler_smooth   wing_small mesh, explicit Euler with RESIDUAL_SMOOTHING= 1, 20 iterations
 
+  Partitioned cases (the reference solver and harness on N MPI ranks, one dump per rank):
+  block8_p4                   the reference's own 4-partition E3D_PRE output of the 8-hex block
+                              (test/mesh/StructuredBlock_8_p0..3.par)
+  naca_small_p3, bump_small_p4  the naca_small / bump_small meshes split by this package's
+                              partitioner (recursive coordinate bisection, precision 16)
+  bump_small_p4_smooth        bump_small_p4 partitions, RK5 with RESIDUAL_SMOOTHING= 1, 10 iterations
+  wing_small_p2_euler_smooth  wing_small in 2 partitions, explicit Euler with smoothing, 20 iterations
+  Each partitioned case stores its partition files as <case>.p<r>.par.gz.
+
 Usage: python tests/golden/make_golden.py [case ...]
 """
 from __future__ import annotations
@@ -105,6 +114,21 @@ for _name, _base, _ti, _sm, _it in (("naca_euler", "naca_small", 0, 0, 150), ("n
                                      ("bump_smooth", "bump_small", 2, 1, 10),
                                      ("wing_euler_smooth", "wing_small", 0, 1, 20)):
     CASES[_name] = dict(CASES[_base], ti=_ti, smooth=_sm, iters=_it, seeds=(), run_mode=True, mesh_of=_base)
+
+
+# Partitioned runs: the reference solver on N MPI ranks (mpiexec -np N) over N partition files.
+# "ref-par:" partitions are the reference's own E3D_PRE output; the others are this package's
+# partitioner (recursive coordinate bisection) applied to another case's mesh.
+PART_CASES = {
+    "block8_p4": dict(CASES["block8"], parts=4, par_source="ref-par:test/mesh/StructuredBlock_8_p#.par", seeds=(0, 1)),
+    "naca_small_p3": dict(CASES["naca_small"], parts=3, mesh_of="naca_small", seeds=(0,)),
+    "bump_small_p4": dict(CASES["bump_small"], parts=4, mesh_of="bump_small", seeds=(0,)),
+    "bump_small_p4_smooth": dict(CASES["bump_small"], parts=4, mesh_of="bump_small", ti=2, smooth=1, iters=10,
+                                 seeds=(), run_mode=True),
+    "wing_small_p2_euler_smooth": dict(CASES["wing_small"], parts=2, mesh_of="wing_small", ti=0, smooth=1, iters=20,
+                                       seeds=(), run_mode=True),
+}
+MPIEXEC = Path("/opt/conda/bin/mpiexec")
 
 
 def parse_dump(path: Path) -> dict:
@@ -215,13 +239,81 @@ def make_case(name: str, case: dict) -> None:
     print(f"wrote {name}: {len(data)} arrays")
 
 
+def partition_texts(name: str, case: dict, td: Path) -> list[str]:
+    """The partition files of a partitioned case, as text."""
+    n = case["parts"]
+    src = case.get("par_source")
+    if src:
+        texts = [(REF / src[8:].replace("#", str(r))).read_text() for r in range(n)]
+        return texts
+    from euler3d_amd import Mesh
+
+    g = td / "global.par"
+    g.write_text(mesh_text(CASES[case["mesh_of"]]))
+    m = Mesh.read(str(g), par=True).preprocess(1.0)
+    paths = [td / f"gen_p{r}.par" for r in range(n)]
+    m.write_partitions([str(p) for p in paths], precision=16)
+    return [p.read_text() for p in paths]
+
+
+def run_harness_ranks(n: int, mode: str, cfg: Path, out: Path, *extra: str) -> list[dict]:
+    r = subprocess.run([str(MPIEXEC), "-np", str(n), str(HARNESS), mode, str(cfg), str(out), *extra],
+                       capture_output=True, text=True, cwd=str(cfg.parent))
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout[-3000:] + r.stderr[-3000:])
+        raise RuntimeError(f"harness {mode} on {n} ranks failed")
+    return [parse_dump(Path(f"{out}.r{k}")) for k in range(n)]
+
+
+def make_part_case(name: str, case: dict) -> None:
+    n = case["parts"]
+    with tempfile.TemporaryDirectory() as td:
+        td = Path(td)
+        (td / "out").mkdir()
+        texts = partition_texts(name, case, td)
+        for r, t in enumerate(texts):
+            (td / f"mesh_p{r}.par").write_text(t)
+        rx, ry, rz = case["ref"]
+        cfg = td / "case.e3d"
+        cfg.write_text(CONFIG.format(mesh="mesh.su2", par=str(td / "mesh_p#.par"), out=str(td / "out"), ocl=case["ocl"],
+                                     ocd=case["ocd"], rx=rx, ry=ry, rz=rz, sref=case["sref"], cref=case["cref"],
+                                     sampling=1, mach=case["mach"], aoa=case["aoa"], cfl=case["cfl"],
+                                     iters=case["iters"], ti=case.get("ti", 2), smooth=case.get("smooth", 0))
+                       .replace("PARTITION_FILES= 1", f"PARTITION_FILES= {n}"))
+        data = {"params": np.array([case["mach"], case["aoa"], 1.4, case["cfl"], case["sref"], case["cref"], rx, ry, rz,
+                                    case["ocl"], case["ocd"], case["iters"]], dtype=np.float64),
+                "integrator": np.array([case.get("ti", 2), case.get("smooth", 0)], dtype=np.int32),
+                "parts": np.array([n], dtype=np.int32)}
+        if not case.get("run_mode"):
+            for r, d in enumerate(run_harness_ranks(n, "mesh", cfg, td / "mesh.txt")):
+                for k, v in d.items():
+                    data[f"r{r}_mesh_{k}"] = v
+            for sd in case["seeds"]:
+                for r, d in enumerate(run_harness_ranks(n, "flux", cfg, td / f"flux{sd}.txt", str(sd))):
+                    for k, v in d.items():
+                        data[f"r{r}_flux{sd}_{k}"] = v
+        for r, d in enumerate(run_harness_ranks(n, "run", cfg, td / "run.txt")):
+            for k, v in d.items():
+                data[f"r{r}_" + (k if k.startswith("run_") else "run_" + k)] = v
+        data["run_residuals_dat"] = parse_residuals(td / "out" / "residuals.dat")
+        data["run_summary_txt"] = np.frombuffer((td / "out" / "summary.txt").read_text().encode(), dtype=np.uint8)
+        np.savez_compressed(HERE / f"{name}.npz", **data)
+        for r, t in enumerate(texts):
+            with gzip.GzipFile(HERE / f"{name}.p{r}.par.gz", "wb", mtime=0) as f:
+                f.write(t.encode())
+    print(f"wrote {name}: {len(data)} arrays")
+
+
 def main(argv: list[str]) -> None:
     if not REF.is_dir():
         raise SystemExit("the reference tree is not present; fixtures are made where it is")
     subprocess.run(["make", "-s", "-f", "oracle/ref.mk", "-j8"], cwd=str(ROOT), check=True)
-    names = argv or list(CASES)
+    names = argv or list(CASES) + list(PART_CASES)
     for n in names:
-        make_case(n, CASES[n])
+        if n in PART_CASES:
+            make_part_case(n, PART_CASES[n])
+        else:
+            make_case(n, CASES[n])
 
 
 if __name__ == "__main__":
