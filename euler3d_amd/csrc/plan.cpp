@@ -134,7 +134,9 @@ HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock) {
 		std::sort(fl, fl + (en - b));
 		for (int j = 0; j < en - b; j++) {
 			int f = fl[j];
-			bool second = sm.face2element[2 * (size_t) f + 1] == old;
+			// the second cell of an interior face subtracts; boundary (and partition interface)
+			// faces only ever add to their own cell
+			bool second = f < sm.nFaceInt && sm.face2element[2 * (size_t) f + 1] == old;
 			P.flist[P.fstart[c] + j] = second ? (int) ((unsigned) f | 0x80000000u) : f;
 		}
 	}

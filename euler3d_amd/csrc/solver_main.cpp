@@ -1,13 +1,14 @@
 // E3D_Solver-compatible executable: `E3D_Solver <config.e3d>`.
 //
 // Same command line, configuration file and output files as the reference's solver main
-// (src/Solver.cpp:19-73), built on the C ABI of libe3d_hip.so. The whole mesh is resident on one
-// GPU, so PARTITION_FILES must be 1 and no MPI launcher is involved.
-//   <outdir>/residuals.dat   include/solver/ResidualsFile.hpp:17-48
-//   <outdir>/cp_part_0.dat   src/solver/EulerSolver.cpp:282-301
-//   <outdir>/summary.txt     src/solver/EulerSolver.cpp:169-198 (TOML; doubles in the shortest
-//                            round-trip form, as toml++ prints them)
-//   <par>.sol, <par>.bry     src/solver/WriteSolution.cpp:5-73
+// (src/Solver.cpp:19-73), built on the C ABI of libe3d_hip.so. Every partition the .e3d file lists
+// is resident on one GPU and solved as the reference's `mpiexec -np <PARTITION_FILES>` run solves
+// it; no MPI launcher is involved. Outputs, per partition r where the reference writes per rank:
+//   <outdir>/residuals.dat     include/solver/ResidualsFile.hpp:17-48
+//   <outdir>/cp_part_<r>.dat   src/solver/EulerSolver.cpp:282-301
+//   <outdir>/summary.txt       src/solver/EulerSolver.cpp:169-198 (TOML; doubles in the shortest
+//                              round-trip form, as toml++ prints them)
+//   <par_r>.sol, <par_r>.bry   src/solver/WriteSolution.cpp:5-73
 #include "../../include/e3d.h"
 
 #include <charconv>
@@ -56,13 +57,29 @@ int main(int argc, char **argv) {
 	if (e3d_config_read(argv[1], &prm, meshPath, sizeof(meshPath), outDir, sizeof(outDir)) != E3D_OK) die("configuration");
 	std::filesystem::path out(outDir);
 
+	int nPart = 1;
+	if (e3d_config_partition(argv[1], 0, nullptr, 0, &nPart) != E3D_OK) die("configuration");
+	std::vector<std::string> parPaths(nPart);
+	for (int r = 0; r < nPart; r++) {
+		char buf[4096];
+		if (e3d_config_partition(argv[1], r, buf, sizeof(buf), nullptr) != E3D_OK) die("configuration");
+		parPaths[r] = buf;
+	}
 	e3d_mesh *mesh = nullptr;
-	if (e3d_mesh_read(meshPath, 1, &mesh) != E3D_OK) die("mesh");
+	if (nPart == 1) {
+		if (e3d_mesh_read(meshPath, 1, &mesh) != E3D_OK) die("mesh");
+	} else {
+		std::vector<const char *> pp;
+		for (auto &x : parPaths) pp.push_back(x.c_str());
+		if (e3d_mesh_read_partitions(nPart, pp.data(), &mesh) != E3D_OK) die("mesh");
+	}
 	if (e3d_mesh_preprocess(mesh, prm.cref) != E3D_OK) die("connectivity/metrics");
 	e3d_mesh_info info;
 	e3d_mesh_get_info(mesh, &info);
-	std::printf("Mesh: %d cells, %d faces (%d interior), %d ghosts (wall %d, farfield %d, sym %d)\n", info.n_elem, info.n_face,
-	            info.n_face_int, info.n_ghost, info.n_wall, info.n_far, info.n_sym);
+	std::printf("Mesh: %d partition(s), %d cells, %d faces (%d interior), %d ghosts (wall %d, farfield %d, sym %d)\n", nPart,
+	            info.n_elem, info.n_face, info.n_face_int, info.n_ghost, info.n_wall, info.n_far, info.n_sym);
+	std::vector<e3d_partition_info> parts(nPart);
+	for (int r = 0; r < nPart; r++) e3d_mesh_partition_info(mesh, r, &parts[r]);
 
 	e3d_ctx *ctx = nullptr;
 	if (e3d_create(mesh, &prm, 0, &ctx) != E3D_OK) die("device context");
@@ -117,47 +134,53 @@ int main(int argc, char **argv) {
 	double vinf = prm.mach * std::sqrt(prm.gamma) * std::sin(prm.aoa_deg * (pi / 180));
 	double cpFactor = 1 / (0.5 * 1.0 * (uinf * uinf + vinf * vinf + 0.0 * 0.0));
 	bool sampled = iters >= prm.sampling;
-	{
-		std::ofstream f(out / "cp_part_0.dat");
-		f << std::setw(20) << "Cp" << std::setw(20) << "x" << std::setw(20) << "y" << std::setw(20) << "z" << std::endl;
-		for (int i = 0; i < info.n_wall; i++) {
-			double cp = sampled ? cpFactor * (fld[4][wallCell[i]] - 1.0) : 0.0;
-			const double *c = &fc[3 * (size_t) wallFace[i]];
-			f << std::scientific << std::setprecision(10) << std::setw(20) << cp << std::setw(20) << c[0] << std::setw(20)
-			  << c[1] << std::setw(20) << c[2] << std::endl;
+	double wsum[3] = {0, 0, 0}; // Wall_Face_Vector_Sum: per partition, then summed in rank order
+	for (int r = 0; r < nPart; r++) {
+		const e3d_partition_info &P = parts[r];
+		{
+			std::ofstream f(out / ("cp_part_" + std::to_string(r) + ".dat"));
+			f << std::setw(20) << "Cp" << std::setw(20) << "x" << std::setw(20) << "y" << std::setw(20) << "z" << std::endl;
+			for (int i = P.wall0; i < P.wall0 + P.n_wall; i++) {
+				double cp = sampled ? cpFactor * (fld[4][wallCell[i]] - 1.0) : 0.0;
+				const double *c = &fc[3 * (size_t) wallFace[i]];
+				f << std::scientific << std::setprecision(10) << std::setw(20) << cp << std::setw(20) << c[0] << std::setw(20)
+				  << c[1] << std::setw(20) << c[2] << std::endl;
+			}
 		}
-	}
-	{
-		std::ofstream f(std::string(meshPath) + ".sol");
-		f << std::setiosflags(std::ios::dec) << std::setiosflags(std::ios::scientific);
-		f << std::setw(20) << "density" << std::setw(20) << "velocity_x" << std::setw(20) << "velocity_y" << std::setw(20)
-		  << "velocity_z" << std::setw(20) << "pressure" << std::setw(20) << "energy" << std::endl;
-		for (int i = 0; i < info.n_elem; i++)
-			f << std::setw(20) << fld[0][i] << std::setw(20) << fld[1][i] << std::setw(20) << fld[2][i] << std::setw(20)
-			  << fld[3][i] << std::setw(20) << fld[4][i] << std::setw(20) << fld[5][i] << std::endl;
-	}
-	{
-		std::ofstream f(std::string(meshPath) + ".bry");
-		f << "NOMBRE DE CELLULES= " << info.n_wall << std::endl;
-		for (int i = 0; i < info.n_wall; i++) {
-			f << std::setw(9) << wallCell[i];
-			int a = f2nStart[wallFace[i]], b = f2nStart[wallFace[i] + 1];
-			f << std::setw(9) << (b - a);
-			for (int j = a; j < b; j++) f << std::setw(9) << f2n[j];
-			f << std::endl;
+		{
+			std::ofstream f(parPaths[r] + ".sol");
+			f << std::setiosflags(std::ios::dec) << std::setiosflags(std::ios::scientific);
+			f << std::setw(20) << "density" << std::setw(20) << "velocity_x" << std::setw(20) << "velocity_y" << std::setw(20)
+			  << "velocity_z" << std::setw(20) << "pressure" << std::setw(20) << "energy" << std::endl;
+			for (int i = P.elem0; i < P.elem0 + P.n_elem; i++)
+				f << std::setw(20) << fld[0][i] << std::setw(20) << fld[1][i] << std::setw(20) << fld[2][i] << std::setw(20)
+				  << fld[3][i] << std::setw(20) << fld[4][i] << std::setw(20) << fld[5][i] << std::endl;
 		}
+		{
+			// the partition's own numbering: local cell ids and local node ids
+			std::ofstream f(parPaths[r] + ".bry");
+			f << "NOMBRE DE CELLULES= " << P.n_wall << std::endl;
+			for (int i = P.wall0; i < P.wall0 + P.n_wall; i++) {
+				f << std::setw(9) << wallCell[i] - P.elem0;
+				int a = f2nStart[wallFace[i]], b = f2nStart[wallFace[i] + 1];
+				f << std::setw(9) << (b - a);
+				for (int j = a; j < b; j++) f << std::setw(9) << f2n[j] - P.node0;
+				f << std::endl;
+			}
+		}
+		double ps[3] = {0, 0, 0};
+		for (int i = P.wall0; i < P.wall0 + P.n_wall; i++)
+			for (int k = 0; k < 3; k++) ps[k] += fn[3 * (size_t) wallFace[i] + k];
+		for (int k = 0; k < 3; k++) wsum[k] = r ? wsum[k] + ps[k] : ps[k];
 	}
 	auto t1 = std::chrono::steady_clock::now();
 	long loopMs = (long) std::chrono::duration_cast<std::chrono::milliseconds>(tRun1 - tRun0).count();
 	long totalMs = (long) std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count();
 	{
-		double wsum[3] = {0, 0, 0};
-		for (int i = 0; i < info.n_wall; i++)
-			for (int k = 0; k < 3; k++) wsum[k] += fn[3 * (size_t) wallFace[i] + k];
-		double perCell = iters > 0 ? (double) (loopMs * 1) / (double) ((long long) iters * info.n_elem) : 0.0;
+		double perCell = iters > 0 ? (double) (loopMs * nPart) / (double) ((long long) iters * info.n_elem) : 0.0;
 		std::ofstream f(out / "summary.txt");
 		f << "Iterations = " << iters << "\n";
-		f << "Ncores = 1\n";
+		f << "Ncores = " << nPart << "\n";
 		f << "SolverLoopTime_ms = " << loopMs << "\n";
 		f << "TimeCore_p_IterNElem_ms = " << tomlDouble(perCell) << "\n";
 		f << "TotalSolverTime_ms = " << totalMs << "\n\n";

@@ -228,10 +228,12 @@ static void face_flux_one(const or_mesh *m, const or_params *q, const or_state *
 	int kind;
 	if (f < m->nFaceInt) {
 		kind = 0;
+	} else if (e2 < m->nElem) {
+		kind = 4; /* partition interface: Roe with the other partition's cell (EulerSolver.cpp:343-346) */
 	} else {
 		kind = ghost_kind_of(m, e2, gk);
 	}
-	if (kind == 0 || kind == 2) {
+	if (kind == 0 || kind == 2 || kind == 4) {
 		if (q->scheme == 1) or_ausm(m, q, s, f, F);
 		else or_roe(m, q, s, f, F);
 	} else if (kind == 1) {
@@ -348,6 +350,15 @@ void or_timestep(const or_mesh *m, const or_params *q, const or_state *s, double
 
 /* FlowField::updateWRungeKutta, FlowField.cpp:140-158, with RHS = alpha*dt*R/V formed as in
  * EulerSolver.cpp:441-449 */
+/* FlowField.cpp:129-138, 159-169: every interface ghost takes a copy of its own cell */
+static void copy_mpi_ghosts(const or_mesh *m, or_state *s) {
+	for (int i = 0; i < m->nMpi; i++) {
+		int g = m->mpiGhost[i], c = m->mpiCell[i];
+		s->rho[g] = s->rho[c], s->u[g] = s->u[c], s->v[g] = s->v[c], s->w[g] = s->w[c];
+		s->E[g] = s->E[c], s->p[g] = s->p[c], s->H[g] = s->H[c], s->M[g] = s->M[c];
+	}
+}
+
 static void rk_update(const or_mesh *m, const or_params *q, or_state *s, const double *W0, const double *R,
                       const double *dt, double alpha) {
 	double g = q->gamma;
@@ -375,6 +386,7 @@ static void rk_update(const or_mesh *m, const or_params *q, or_state *s, const d
 		s->H[e] = H;
 		s->M[e] = M;
 	}
+	copy_mpi_ghosts(m, s);
 }
 
 /* EulerSolver::RungeKutta, EulerSolver.cpp:425-472 */
@@ -412,11 +424,32 @@ double or_rms_sum(const double *R, int nElem) {
 
 /* AeroCoefficients ctor + Update, AeroCoefficients.cpp:5-52. The centre of pressure is never
  * divided (Vector3::operator/=(Vector3) returns a new vector, Vector3.h:107-109). */
+static void aero_range(const or_mesh *m, const or_params *q, const or_state *s, int w0, int w1, double cf[3],
+                       double cm[3]);
+
+/* With several partitions each computes its own force and moment vectors, and rank 0 adds them
+ * in rank order (MPIHandler::UpdateAerodynamicCoefficients, MPIHandler.cpp:268-313). */
 void or_aero(const or_mesh *m, const or_params *q, const or_state *s, double cf[3], double cm[3]) {
+	if (m->nPart <= 1) {
+		aero_range(m, q, s, 0, m->nWall, cf, cm);
+		return;
+	}
+	for (int r = 0; r < m->nPart; r++) {
+		double f[3], c[3];
+		aero_range(m, q, s, m->partWallStart[r], m->partWallStart[r + 1], f, c);
+		for (int k = 0; k < 3; k++) {
+			cf[k] = r ? cf[k] + f[k] : f[k];
+			cm[k] = r ? cm[k] + c[k] : c[k];
+		}
+	}
+}
+
+static void aero_range(const or_mesh *m, const or_params *q, const or_state *s, int w0, int w1, double cf[3],
+                       double cm[3]) {
 	double uinf2 = q->u_inf * q->u_inf + q->v_inf * q->v_inf + q->w_inf * q->w_inf;
 	double cpFactor = 1 / (0.5 * q->rho_inf * uinf2);
 	double F[3] = {0, 0, 0}, CP[3] = {0, 0, 0};
-	for (int i = 0; i < m->nWall; i++) {
+	for (int i = w0; i < w1; i++) {
 		int cell = m->wallCell[i];
 		int f = m->wallFace[i];
 		double cp = cpFactor * (s->p[cell] - q->p_inf);
@@ -442,7 +475,24 @@ void or_aero(const or_mesh *m, const or_params *q, const or_state *s, double cf[
  * writes its changes through begin() of the empty vector and scans zero elements, so the second
  * pass always reports no change: one sweep when the first changes nothing, else exactly two.
  * Only the first pass checks for NaN (ResidualVar::findMax exits the process on one). */
+static int smooth_range(const or_mesh *m, const or_params *q, double *R, size_t e0, size_t e1);
+
+/* Several partitions: each rank runs its own loop over its own cells (interface neighbours are
+ * ghosts there), so each partition stops on its own first sweep. The return value is the largest
+ * number of sweeps, or -1 on a NaN. */
 int or_smooth(const or_mesh *m, const or_params *q, double *R) {
+	if (m->nPart <= 1) return smooth_range(m, q, R, 0, (size_t) m->nElem);
+	int most = 0;
+	for (int r = 0; r < m->nPart; r++) {
+		int k = smooth_range(m, q, R, (size_t) m->partElemStart[r], (size_t) m->partElemStart[r + 1]);
+		if (k < 0) return -1;
+		if (k > most) most = k;
+	}
+	return most;
+}
+
+/* one partition's sweeps over cells [e0, e1) (whose interior neighbours are all in that range) */
+static int smooth_range(const or_mesh *m, const or_params *q, double *R, size_t e0, size_t e1) {
 	const double epsilon = 0.25;
 	const size_t n = (size_t) m->nElem;
 	double *orig = (double *) malloc(sizeof(double) * 5 * n);
@@ -453,7 +503,7 @@ int or_smooth(const or_mesh *m, const or_params *q, double *R) {
 	do {
 		error = 0.0;
 		memcpy(last, R, sizeof(double) * 5 * n);
-		for (size_t e = 0; e < n; e++) {
+		for (size_t e = e0; e < e1; e++) {
 			double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 			int b = m->element2faceStart[e], en = m->element2faceStart[e + 1];
 			int ns = en - b;
@@ -466,7 +516,7 @@ int or_smooth(const or_mesh *m, const or_params *q, double *R) {
 			for (int k = 0; k < 5; k++) R[5 * e + k] = (orig[5 * e + k] + sum[k]) / den;
 		}
 		if (sweeps == 0) {
-			for (size_t i = 0; i < 5 * n; i++) {
+			for (size_t i = 5 * e0; i < 5 * e1; i++) {
 				double d = fabs(last[i] - R[i]);
 				if (isnan(d)) {
 					free(orig);
@@ -526,6 +576,7 @@ void or_euler(const or_mesh *m, const or_params *q, or_state *s, double *W, cons
 		s->H[e] = H;
 		s->M[e] = M;
 	}
+	copy_mpi_ghosts(m, s);
 	free(dt);
 }
 

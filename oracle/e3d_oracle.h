@@ -22,6 +22,9 @@
  *   or_euler           EulerSolver.cpp:419-423, TimeIntegration.cpp:78-91, FlowField.cpp:103-126
  *                                                  golden fixtures (runs with TIME_INTEGRATION= 0)
  *   or_run_any         EulerSolver.cpp:60-167      golden fixtures (TIME_INTEGRATION 0/1/2, smoothing)
+ *   partitioned meshes (nPart > 1): interface faces, per-partition coefficients and smoothing
+ *                      stops (MPIHandler.cpp:16-313, FlowField.cpp:129-169)
+ *                                                  golden fixtures of mpiexec -np N runs (r<r>_*)
  *   or_ausm            no reference implementation exists (ConvectiveFluxesSchemes.h:27-30 only
  *                      declares it): PARITY UNPINNED. Restates Liou & Steffen, J. Comput. Phys.
  *                      107 (1993) 23-39, and is checked by invariants only (consistency
@@ -48,6 +51,14 @@ typedef struct {
 	const int *wallGhost, *wallCell, *wallFace;
 	const int *farGhost, *farCell, *farFace;
 	const int *symGhost, *symCell, *symFace;
+	/* several partitions as one mesh (zero / NULL otherwise): interface faces are boundary faces
+	 * whose elem1 is the other partition's cell; each interface ghost copies its own cell after an
+	 * update; cells [partElemStart[r], partElemStart[r+1]) and wall entries [partWallStart[r], ...)
+	 * belong to partition r */
+	int nMpi;
+	const int *mpiGhost, *mpiCell;
+	int nPart;
+	const int *partElemStart, *partWallStart;
 } or_mesh;
 
 typedef struct {

@@ -25,7 +25,9 @@ class OrMesh(C.Structure):
                 ("cellVolume", DP), ("nWall", C.c_int), ("nFar", C.c_int), ("nSym", C.c_int),
                 ("wallGhost", IP), ("wallCell", IP), ("wallFace", IP),
                 ("farGhost", IP), ("farCell", IP), ("farFace", IP),
-                ("symGhost", IP), ("symCell", IP), ("symFace", IP)]
+                ("symGhost", IP), ("symCell", IP), ("symFace", IP),
+                ("nMpi", C.c_int), ("mpiGhost", IP), ("mpiCell", IP),
+                ("nPart", C.c_int), ("partElemStart", IP), ("partWallStart", IP)]
 
 
 class OrParams(C.Structure):
@@ -103,9 +105,13 @@ class Oracle:
         f2e = np.asarray(arrays["face2element"])
         m.nFace = len(f2e) // 2
         m.nElem = len(arrays["cellVolumes"])
-        m.nGhost = int(f2e[1::2].max()) + 1 - m.nElem if m.nFace else 0
-        nbnd = int(np.count_nonzero(f2e[1::2] >= m.nElem))
-        m.nFaceInt = m.nFace - nbnd
+        if "nFaceInt" in arrays:  # partitioned meshes: interface faces have an interior elem1
+            m.nFaceInt = int(arrays["nFaceInt"])
+            m.nGhost = m.nFace - m.nFaceInt
+        else:
+            m.nGhost = int(f2e[1::2].max()) + 1 - m.nElem if m.nFace else 0
+            nbnd = int(np.count_nonzero(f2e[1::2] >= m.nElem))
+            m.nFaceInt = m.nFace - nbnd
         for name, field in (("face2element", "face2element"), ("element2faceStart", "element2faceStart"),
                             ("element2face", "element2face")):
             a, ptr = _i(arrays[name])
@@ -123,6 +129,18 @@ class Oracle:
                 self._keep.append(a)
                 setattr(m, grp + part, ptr)
             setattr(m, "n" + grp.capitalize(), len(arrays[grp + "Ghost"]))
+        if len(arrays.get("mpiGhost", ())):
+            for name in ("mpiGhost", "mpiCell"):
+                a, ptr = _i(arrays[name])
+                self._keep.append(a)
+                setattr(m, name, ptr)
+            m.nMpi = len(arrays["mpiGhost"])
+        if "partElemStart" in arrays and len(arrays["partElemStart"]) > 2:
+            for name in ("partElemStart", "partWallStart"):
+                a, ptr = _i(arrays[name])
+                self._keep.append(a)
+                setattr(m, name, ptr)
+            m.nPart = len(arrays["partElemStart"]) - 1
         self.mesh = m
         q = OrParams()
         q.gamma = params.get("gamma", 1.4)

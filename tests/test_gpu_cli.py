@@ -80,3 +80,48 @@ def test_cli_matches_reference_outputs(case, tmp_path):
     vals = np.array(lines[4 + 3 * nn:4 + 3 * nn + 6 * n], dtype=np.float64).reshape(6, n).T
     np.testing.assert_array_equal(vals, sol)
     assert len(lines) == 4 + 3 * nn + 7 * n
+
+
+@pytest.mark.parametrize("case", ["block8_p4", "naca_small_p3", "bump_small_p4_smooth", "wing_small_p2_euler_smooth"])
+def test_cli_partitioned_matches_reference_outputs(case, tmp_path):
+    """PARTITION_FILES= N: every partition resident on the one device, outputs as the reference's
+    `mpiexec -np N` run writes them (residuals.dat, summary.txt with Ncores = N, per-partition
+    .sol and cp_part_<r>.dat)."""
+    import gzip
+
+    print(f"[gpu] E3D_Solver partitioned case={case}", flush=True)
+    g = load_golden(case)
+    pd = integrator_params(g)
+    p = g["params"]
+    n = int(g["parts"][0])
+    for r in range(n):
+        (tmp_path / f"mesh_p{r}.par").write_bytes(gzip.decompress((GOLDEN / f"{case}.p{r}.par.gz").read_bytes()))
+    (tmp_path / "out").mkdir()
+    cfg = tmp_path / "case.e3d"
+    rx, ry, rz = pd["ref_point"]
+    cfg.write_text(CONFIG.format(mesh="mesh.su2", par=str(tmp_path / "mesh_p#.par"), out=str(tmp_path / "out"),
+                                 ocl=int(p[9]), ocd=int(p[10]), rx=rx, ry=ry, rz=rz, sref=pd["sref"], cref=pd["cref"],
+                                 sampling=1, mach=pd["mach"], aoa=pd["aoa_deg"], cfl=pd["cfl"], iters=pd["max_iter"],
+                                 ti=pd["time_integration"], smooth=pd["residual_smoothing"])
+                   .replace("PARTITION_FILES= 1", f"PARTITION_FILES= {n}"))
+    r = subprocess.run([str(EXE), str(cfg)], cwd=str(tmp_path), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    rows = np.loadtxt(tmp_path / "out" / "residuals.dat", skiprows=1, ndmin=2)
+    rd = g["run_residuals_dat"]
+    assert rows.shape == rd.shape
+    np.testing.assert_array_equal(rows[:, 0], rd[:, 0])
+    np.testing.assert_allclose(rows[:, 1:], rd[:, 1:], rtol=6e-11, atol=1e-13)
+    ours = summary_values((tmp_path / "out" / "summary.txt").read_text())
+    ref = summary_values(bytes(g["run_summary_txt"]).decode())
+    assert ours["Iterations"] == ref["Iterations"] and ours["NElem"] == ref["NElem"] and ours["Ncores"] == n
+    for k in ("CL", "CD", "CM"):
+        assert abs(ours[k] - ref[k]) <= 1e-12 * max(abs(ref[k]), 1e-3), (k, ours[k], ref[k])
+    for rank in range(n):
+        sol = np.loadtxt(str(tmp_path / f"mesh_p{rank}.par") + ".sol", skiprows=1, ndmin=2)
+        ne = len(g[f"r{rank}_run_residual"]) // 5
+        assert sol.shape == (ne, 6)
+        for col, k in enumerate(("rho", "u", "v", "w", "p", "E")):
+            expect = np.array([float("%e" % x) for x in g[f"r{rank}_run_{k}"][:ne]])
+            np.testing.assert_array_equal(sol[:, col], expect, err_msg=f"{k} partition {rank}")
+        assert (tmp_path / "out" / f"cp_part_{rank}.dat").exists()
+        assert (tmp_path / f"mesh_p{rank}.par.bry").exists()

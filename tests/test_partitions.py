@@ -242,3 +242,112 @@ def test_partitioner_round_trip(n_part, tmp_path):
     assert ci["n_face_int"] + n_mpi // 2 == gi["n_face_int"]
     assert n_mpi % 2 == 0
     np.testing.assert_allclose(np.sort(cm.get("cellVolumes")), np.sort(gm.get("cellVolumes")), rtol=1e-9)
+
+
+# ---- the oracle on the composite mesh against the reference's ranks
+
+FIELDS8 = ("rho", "u", "v", "w", "p", "E", "H", "M")
+
+
+def composite_arrays(m: Mesh) -> dict:
+    """Oracle input for a partitioned mesh (the product's preprocessing)."""
+    from conftest import MESH_KEYS_I
+
+    arr = {k: m.get(k) for k in MESH_KEYS_I + MESH_KEYS_F}
+    parts = m.partitions()
+    arr["nFaceInt"] = m.info()["n_face_int"]
+    arr["mpiGhost"], arr["mpiCell"] = m.get("mpiGhost"), m.get("mpiCell")
+    arr["partElemStart"] = m.get("partStart")
+    arr["partWallStart"] = np.array([q["wall0"] for q in parts] + [parts[-1]["wall0"] + parts[-1]["n_wall"]])
+    return arr
+
+
+def join_cells(g: dict, key: str, parts) -> np.ndarray:
+    """Per-rank arrays over (interior cells, ghosts) -> the composite's (all interiors, all ghosts)."""
+    inner = [g[f"r{r}_{key}"][:q["n_elem"]] for r, q in enumerate(parts)]
+    ghost = [g[f"r{r}_{key}"][q["n_elem"]:q["n_elem"] + q["n_ghost"]] for r, q in enumerate(parts)]
+    return np.concatenate(inner + ghost)
+
+
+def join_interior(g: dict, key: str, parts, width: int = 1) -> np.ndarray:
+    return np.concatenate([g[f"r{r}_{key}"][:width * q["n_elem"]] for r, q in enumerate(parts)])
+
+
+def split_faces(values: np.ndarray, parts, n_face_int: int, width: int) -> list:
+    """composite per-face values -> per-rank arrays in each rank's face order"""
+    v = values.reshape(-1, width)
+    out = []
+    for q in parts:
+        idx = np.concatenate([np.arange(q["face_int0"], q["face_int0"] + q["n_face_int"]),
+                              n_face_int + q["ghost0"] + np.arange(q["n_ghost"])])
+        out.append(v[idx].ravel())
+    return out
+
+
+def oracle_params(g: dict) -> dict:
+    from conftest import integrator_params
+
+    return integrator_params(g)
+
+
+@pytest.mark.parametrize("case", PART_CASES)
+def test_oracle_partitioned_flux_and_step(case, tmp_path):
+    from oracle_ctypes import Oracle
+
+    g = load_golden(case)
+    m = composite(case, tmp_path)
+    parts = m.partitions()
+    nFI = m.info()["n_face_int"]
+    mpi_ghost = set(m.get("mpiGhost").tolist())
+    nE = m.info()["n_elem"]
+    seeds = sorted({int(k.split("_")[1][4:]) for k in g if k.startswith("r0_flux") and k.endswith("_in_rho")})
+    for sd in seeds:
+        o = Oracle(composite_arrays(m), oracle_params(g))
+        o.set_state({k: join_cells(g, f"flux{sd}_in_{k}", parts) for k in FIELDS8})
+        o.update_bc()
+        keep = np.array([i for i in range(o.n_tot) if i not in mpi_ghost])
+        for k in FIELDS8:
+            want = join_cells(g, f"flux{sd}_bc_{k}", parts)
+            np.testing.assert_array_equal(o.state[k][keep].view(np.uint64), want[keep].view(np.uint64), err_msg=k)
+        F, FS, kind = o.face_flux()
+        for r, (f, fs, kd) in enumerate(zip(split_faces(F, parts, nFI, 5), split_faces(FS, parts, nFI, 5),
+                                            split_faces(kind, parts, nFI, 1))):
+            np.testing.assert_array_equal(kd, g[f"r{r}_flux{sd}_face_kind"])
+            np.testing.assert_array_equal(f.view(np.uint64), g[f"r{r}_flux{sd}_face_flux"].view(np.uint64))
+            np.testing.assert_array_equal(fs.view(np.uint64), g[f"r{r}_flux{sd}_face_flux_scaled"].view(np.uint64))
+        R = o.residual()
+        np.testing.assert_array_equal(R.view(np.uint64), join_interior(g, f"flux{sd}_residual", parts, 5).view(np.uint64))
+        dt = o.timestep()
+        np.testing.assert_array_equal(dt.view(np.uint64), join_interior(g, f"flux{sd}_dt", parts).view(np.uint64))
+        o.rk5()
+        for k in FIELDS8:
+            want = join_cells(g, f"flux{sd}_rk_{k}", parts)
+            np.testing.assert_array_equal(o.state[k].view(np.uint64), want.view(np.uint64), err_msg=f"rk {k}")
+        total = sum(float(g[f"r{r}_flux{sd}_rk_rms_sum"][0]) for r in range(len(parts)))
+        assert abs(o.rms_sum() - total) <= 1e-13 * abs(total)
+    assert nE == sum(q["n_elem"] for q in parts)
+
+
+@pytest.mark.parametrize("case", PART_CASES + ("bump_small_p4_smooth", "wing_small_p2_euler_smooth"))
+def test_oracle_partitioned_run(case, tmp_path):
+    """A full multi-rank run: final states bit for bit (per partition), the coefficients summed
+    per partition in rank order, the residual history to the RMS tolerance."""
+    from oracle_ctypes import Oracle
+
+    g = load_golden(case)
+    m = composite(case, tmp_path)
+    parts = m.partitions()
+    pd = oracle_params(g)
+    arr = composite_arrays(m)
+    o = Oracle(arr, pd)
+    out = o.run()
+    for k in FIELDS8:
+        want = join_cells(g, f"run_{k}", parts)
+        np.testing.assert_array_equal(o.state[k].view(np.uint64), want.view(np.uint64), err_msg=k)
+    assert out["iterations"] == int(g["r0_run_iterations"][0])
+    for k in ("CL", "CD", "CM"):
+        ref = float(g[f"r0_run_{k}"][0])
+        assert out[k] == ref or abs(out[k] - ref) <= 1e-13 * max(abs(ref), 1e-12), (k, out[k], ref)
+    rd = g["run_residuals_dat"]
+    assert out["rows"].shape == rd.shape
+    np.testing.assert_allclose(out["rows"][:, 1:], rd[:, 1:], rtol=6e-11, atol=1e-13)
