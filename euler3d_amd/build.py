@@ -1,5 +1,5 @@
 """Build recipe of the native parts: libe3d_hip.so (HIP kernels for gfx950 + host C++), the
-E3D_Solver- and E3D_POST-compatible executables, and the CPU oracle used by the tests.
+E3D_PRE-, E3D_Solver- and E3D_POST-compatible executables, and the CPU oracle used by the tests.
 
 Run as ``python -m euler3d_amd.build`` or through ``__graft_entry__.build()``. Outputs go next to
 the sources (in-tree) so that they travel with the repository snapshot:
@@ -7,6 +7,7 @@ the sources (in-tree) so that they travel with the repository snapshot:
     euler3d_amd/lib/libe3d_hip.so
     euler3d_amd/bin/E3D_Solver
     euler3d_amd/bin/E3D_POST
+    euler3d_amd/bin/E3D_PRE
     oracle/lib/libe3d_oracle.so          (test infrastructure)
 
 Host C++ is compiled with g++ -O2 -ffp-contract=off (the arithmetic of the mesh metrics must
@@ -31,7 +32,7 @@ ARCH = os.environ.get("E3D_OFFLOAD_ARCH", "gfx950")
 
 HOST_SRCS = ["mesh_io.cpp", "config.cpp", "preprocess.cpp", "partitions.cpp", "generators.cpp", "plan.cpp", "post.cpp",
              "capi.cpp", "batch.cpp"]
-MAINS = {"E3D_Solver": "solver_main.cpp", "E3D_POST": "post_main.cpp"}
+MAINS = {"E3D_Solver": "solver_main.cpp", "E3D_POST": "post_main.cpp", "E3D_PRE": "pre_main.cpp"}
 DEVICE_SRCS = ["kernels.hip", "kernels_mixed.hip"]
 
 CXXFLAGS = ["-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",

@@ -296,6 +296,16 @@ int e3d_config_partition(const char *path, int index, char *buf, int len, int *n
 	});
 }
 
+int e3d_config_initial_mesh(const char *path, char *buf, int len) {
+	if (!path || !buf || len <= 0) return fail(E3D_ERR_ARG, "null argument");
+	return guarded([&] {
+		e3d::Params p = e3d::readConfig(path);
+		std::strncpy(buf, p.initialMesh.c_str(), (size_t) len - 1);
+		buf[len - 1] = 0;
+		return E3D_OK;
+	});
+}
+
 int e3d_mesh_read(const char *path, int require_par, e3d_mesh **out) {
 	if (!path || !out) return fail(E3D_ERR_ARG, "null argument");
 	return guarded([&] {

@@ -68,6 +68,7 @@ _SIGS = {
     "e3d_params_default": (C.c_int, [C.POINTER(Params)]),
     "e3d_config_read": (C.c_int, [C.c_char_p, C.POINTER(Params), C.c_char_p, C.c_int, C.c_char_p, C.c_int]),
     "e3d_config_partition": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p, C.c_int, IP]),
+    "e3d_config_initial_mesh": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int]),
     "e3d_mesh_read": (C.c_int, [C.c_char_p, C.c_int, PP]),
     "e3d_mesh_read_partitions": (C.c_int, [C.c_int, C.POINTER(C.c_char_p), PP]),
     "e3d_mesh_partition_count": (C.c_int, [P, IP]),

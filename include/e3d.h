@@ -91,9 +91,11 @@ int e3d_params_default(e3d_params *out);
  * '#' -> 0), out_dir the directory of OUTPUT_FILE. */
 int e3d_config_read(const char *path, e3d_params *out, char *mesh_path, int mesh_path_len, char *out_dir,
                     int out_dir_len);
-/* SimConfig::getPartitionedMeshFiles (SimConfig.cpp:52-66): partition file `index` of the .e3d
+/* SimConfig::getPartitionedMeshFiles (SimConfig.cpp:66-87): partition file `index` of the .e3d
  * file into buf (may be NULL) and the partition count into *n_part (may be NULL). */
 int e3d_config_partition(const char *path, int index, char *buf, int len, int *n_part);
+/* INITIAL_MESH of the .e3d file (SimConfig::getInitialMeshFile, parsed at SimConfig.cpp:64-65) */
+int e3d_config_initial_mesh(const char *path, char *buf, int len);
 
 /* ---- meshes ------------------------------------------------------------------------------- */
 /* SU2 (.su2) or partition (.par, require_par = 1: the NINTERNAL_MARKER block must be present) */

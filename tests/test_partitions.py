@@ -351,3 +351,42 @@ def test_oracle_partitioned_run(case, tmp_path):
     rd = g["run_residuals_dat"]
     assert out["rows"].shape == rd.shape
     np.testing.assert_allclose(out["rows"][:, 1:], rd[:, 1:], rtol=6e-11, atol=1e-13)
+
+
+def test_e3d_pre_executable(tmp_path):
+    """`E3D_PRE <case.e3d>` (src/preprocessor.cpp:16-45): INITIAL_MESH split into the
+    PARTITION_FILES, the SU2Writer layout at its default 10 digits; the same bytes as the library
+    writer, and the files read back as one mesh of the same cells."""
+    import subprocess
+    import sys
+
+    from conftest import ROOT
+
+    sys.path.insert(0, str(GOLDEN))
+    from make_golden import CONFIG
+
+    src = tmp_path / "naca.su2"
+    src.write_bytes(gzip.decompress((GOLDEN / "naca_small.par.gz").read_bytes()))
+    for n in (1, 3):
+        d = tmp_path / f"n{n}"
+        d.mkdir()
+        cfg = d / "case.e3d"
+        cfg.write_text(CONFIG.format(mesh=str(src), par=str(d / "part_#.par"), out=str(d), ocl=2, ocd=0, rx=0.0, ry=0.0,
+                                     rz=0.0, sref=1.0, cref=1.0, sampling=1, mach=0.5, aoa=0.0, cfl=1.0, iters=1, ti=2,
+                                     smooth=0).replace("PARTITION_FILES= 1", f"PARTITION_FILES= {n}"))
+        r = subprocess.run([str(ROOT / "euler3d_amd" / "bin" / "E3D_PRE"), str(cfg)], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        gm = Mesh.read(str(src)).preprocess(1.0)
+        lib_paths = [d / f"lib_{k}.par" for k in range(n)]
+        gm.write_partitions([str(p) for p in lib_paths], precision=10)
+        for k in range(n):
+            assert (d / f"part_{k}.par").read_bytes() == lib_paths[k].read_bytes()
+        if n == 1:
+            assert "NINTERNAL_MARKER= 0\n" in (d / "part_0.par").read_text()
+            assert Mesh.read(str(d / "part_0.par")).preprocess(1.0).info()["n_elem"] == gm.info()["n_elem"]
+        else:
+            cm = Mesh.read_partitions([str(d / f"part_{k}.par") for k in range(n)]).preprocess(1.0)
+            assert cm.info()["n_elem"] == gm.info()["n_elem"]
+    r = subprocess.run([str(ROOT / "euler3d_amd" / "bin" / "E3D_PRE")], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "Usage" in r.stderr

@@ -136,7 +136,7 @@ def test_config_errors(tmp_path):
         SimConfig.read(str(p))
     p.write_text("START\nPARTITION_FILES= 2\np#.par\nMESH_ORIENTATION_CL= 2\nMESH_ORIENTATION_CD= 0\n"
                  "SPEED_VALUE= 0.5\nSAMPLING= 1\nCFL= 1\nMAX_ITER= 1\nOUTPUT_FILE= a.dat\nEND\n")
-    c = SimConfig.read(str(p))  # several partitions: every file of the pattern (SimConfig.cpp:52-66)
+    c = SimConfig.read(str(p))  # several partitions: every file of the pattern (SimConfig.cpp:66-87)
     assert c.partition_files == ("p0.par", "p1.par") and c.mesh_path == "p0.par"
     with pytest.raises(E3DError):
         SimConfig.read(str(tmp_path / "missing.e3d"))
