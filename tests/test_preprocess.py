@@ -199,3 +199,4 @@ def test_wing_generator_cells_are_closed(dims):
     np.add.at(A, f2e[:, 0], m.get("faceSurfaces"))
     assert np.max(np.linalg.norm(C, axis=1) / A) < 1e-12
     assert np.all(m.get("cellVolumes") > 0)
+
