@@ -405,7 +405,12 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 	char *mem = nullptr;
 	int rc = guarded([&] {
 		hipCheck(hipSetDevice(b->device), "hipSetDevice");
-		e3d::HostPlan P = e3d::buildPlan(sm, e3d::ORDER_MORTON, e3d::kMixBlock);
+		int order = e3d::ORDER_BISECT; // box-shaped blocks cut fewer faces than Morton runs
+		if (const char *v = std::getenv("E3D_CELL_ORDER")) {
+			if (std::string(v) == "morton") order = e3d::ORDER_MORTON;
+			if (std::string(v) == "reference") order = e3d::ORDER_REFERENCE;
+		}
+		e3d::HostPlan P = e3d::buildPlan(sm, order, e3d::kMixBlock);
 		if (P.csrStride > 6) return fail(E3D_ERR_UNSUPPORTED, "cells with more than 6 faces are not on the device path");
 		std::vector<int> blockInt;
 		const int maxInt = interiorFirst(P, blockInt, p->scheme);
