@@ -245,6 +245,56 @@ def bench_serve(a) -> list[dict]:
     return lines
 
 
+def bench_wing(a) -> dict:
+    """The swept tapered wing of the accuracy case, 193x97x65 nodes (1,179,648 hexes), M 0.84,
+    AoA 3 deg: the FP64 path and the mixed-precision engine (FP32 flux and BC, FP64 update with
+    compensated accumulation) run the same RK5 iterations from freestream. Reported: CL, CD and the
+    density of both, their differences, and each path's throughput."""
+    import numpy as np
+
+    from euler3d_amd import Batch, EulerSolver, Mesh, default_params
+
+    t = time.perf_counter()
+    m = Mesh.wing(ni=193, nj=97, nk=65, thickness=0.10, sweep_deg=30.0, taper=0.56, semispan=1.5, farfield=15.0)
+    m.preprocess(1.0)
+    n = m.info()["n_elem"]
+    log(f"wing: {n} cells, mesh in {time.perf_counter() - t:.1f} s")
+    iters = a.max_iter
+    prm = default_params(mach=0.84, aoa_deg=3.0, gamma=1.4, cfl=1.5, sref=1.0, cref=1.0, max_iter=iters,
+                         orient_cl=(1, 1), orient_cd=(0, 1), orient_cm=(2, 1))
+    with EulerSolver(m, prm) as s:
+        t0 = time.perf_counter()
+        r64 = s.run()
+        wall64 = time.perf_counter() - t0
+        rho64 = s.get_state()["rho"][:n]
+    log(f"fp64: {r64['iterations']} iterations in {wall64:.2f} s, CL {r64['CL']:.9f} CD {r64['CD']:.9f}")
+    with Batch(max_cases=1) as b:
+        cid = b.add(m, prm, drop=0.0, max_iter=iters)
+        t0 = time.perf_counter()
+        while not b.poll()[0]["done"]:
+            b.run(200)
+        wall32 = time.perf_counter() - t0
+        res = b.result(cid)
+        rho32 = b.get_state(cid)["rho"][:n]
+    log(f"mixed: {len(res['rms'])} iterations in {wall32:.2f} s, CL {res['CL']:.9f} CD {res['CD']:.9f}")
+    rows = r64["rows"]
+    h32 = res["rms"]
+    return {"metric": "mixed-precision error vs FP64 (wing)", "value": abs(res["CL"] - r64["CL"]), "unit": "|dCL|",
+            "n_gpus": 1, "higher_is_better": False, "dtype": "f32 flux / f64 update vs f64", "data": "synthetic",
+            "config": {"workload": "swept tapered wing, 10% symmetric 4-digit section, LE sweep 30 deg, taper 0.56, "
+                                   "semispan 1.5 root chords, O-H grid 193x97x65 nodes, farfield 15 root chords, "
+                                   f"M 0.84, AoA 3 deg, Roe, RK5 CFL 1.5, {iters} iterations from freestream",
+                       "cells": n},
+            "fp64": {"CL": r64["CL"], "CD": r64["CD"], "iterations": r64["iterations"], "rms_first": float(rows[0, 1]),
+                     "rms_final": float(rows[-1, 1]), "rms_floor": float(rows[:, 1].min()),
+                     "wall_s": wall64, "cell_stage_per_s": 5 * n * r64["iterations"] / wall64},
+            "mixed": {"CL": res["CL"], "CD": res["CD"], "iterations": len(h32), "rms_first": float(h32[0]),
+                      "rms_final": float(h32[-1]), "rms_floor": float(h32.min()),
+                      "wall_s": wall32, "cell_stage_per_s": 5 * n * len(res["rms"]) / wall32},
+            "abs_dCL": abs(res["CL"] - r64["CL"]), "abs_dCD": abs(res["CD"] - r64["CD"]),
+            "linf_drho": float(np.max(np.abs(rho32 - rho64)))}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,9 +309,9 @@ def main() -> None:
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run one plain device-to-device copy of this many MiB (a known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--mode", choices=("plain", "batch", "serve"), default="plain",
+    ap.add_argument("--mode", choices=("plain", "batch", "serve", "wing"), default="plain",
                     help="plain: the FP64 stage benchmark (default); batch: 64 uneven airfoil cases; "
-                         "serve: Poisson request streams")
+                         "serve: Poisson request streams; wing: mixed precision against FP64 on the wing")
     ap.add_argument("--cases", type=int, default=64)
     ap.add_argument("--requests", type=int, default=24)
     ap.add_argument("--max-cells", type=float, default=2e6)
@@ -280,6 +330,9 @@ def main() -> None:
         return
     if a.mode == "serve":
         bench_serve(a)
+        return
+    if a.mode == "wing":
+        print(json.dumps(bench_wing(a)), flush=True)
         return
 
     from euler3d_amd import EulerSolver, Mesh, default_params
