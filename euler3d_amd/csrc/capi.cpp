@@ -156,7 +156,7 @@ void runStage(e3d_ctx *c, int stage, double *rmsPartial) {
 		auto o = c->state(c->cur ^ 1);
 		record(evA);
 		e3d::launchStageFused(c->dm, c->plan, s, o, c->fr, c->prm.scheme, ti == 0 ? 1 : 0, c->W0.p, alphas[stage],
-		                      stage == 0, c->machDerived, rmsPartial, c->tune, c->stream);
+		                      stage == 0, c->machDerived, rmsPartial, c->cur, c->tune, c->stream);
 		record(evB);
 		c->cur ^= 1;
 	} else {
@@ -624,8 +624,9 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		if (const char *v = std::getenv("E3D_NONTEMPORAL")) c->tune.nontemporal = std::atoi(v) != 0;
 		if (const char *v = std::getenv("E3D_XCDS")) c->tune.xcds = std::max(1, std::atoi(v));
 		if (const char *v = std::getenv("E3D_SHARED_RECIP")) c->tune.sharedRecip = std::atoi(v) != 0;
+		if (const char *v = std::getenv("E3D_ALTERNATE")) c->tune.alternate = std::atoi(v) != 0;
 		e3d::HostPlan P = e3d::buildPlan(sm, order, e3d::kCellsPerBlock);
-		if (((size_t) P.maxSlots * 5 + (size_t) 6 * (e3d::kCellsPerBlock + P.maxHalo)) * sizeof(double) > 160 * 1024)
+		if (e3d::stageFusedLdsBytes(P.maxSlots, P.maxHalo) > 160 * 1024)
 			return fail(E3D_ERR_UNSUPPORTED, "a block of the fused stage needs more LDS than a CU has");
 		c->new2old = P.new2old;
 

@@ -94,6 +94,8 @@ void launchStage(const DevMesh &m, DevState s, const DevFree &q, const double *f
 void launchSmooth(const double *R0, double *R1, double *R2, const int *nbr, const uint8_t *nface, int stride, int nElem,
                   double *blockMax, int *flag, const int *partStart, int nPart, double *cellMax, hipStream_t st);
 int stageBlocks(int nElem);
+// dynamic LDS of one fused-stage workgroup for a plan's largest block
+size_t stageFusedLdsBytes(int maxSlots, int maxHalo);
 // launch choices of the fused stage (they change speed only, never results)
 struct StageTuning {
 	bool nontemporal = true; // non-temporal loads for arrays read once per stage
@@ -101,6 +103,9 @@ struct StageTuning {
 	// quotients sharing a denominator use one IEEE reciprocal plus Markstein's FMA correction,
 	// which is correctly rounded and so bit-identical to the divisions it replaces
 	bool sharedRecip = true;
+	// every other stage walks each XCD's block range backwards, so that it starts on the cells
+	// the previous stage wrote last, which are still in the memory-side cache
+	bool alternate = true;
 };
 
 // Fused RK stage: boundary ghosts, face fluxes (once per block, in LDS), residual gather, local
@@ -108,9 +113,10 @@ struct StageTuning {
 // after a state upload (machDerived == 0) the interior Mach number comes from the state's M slot.
 // integ 0: RK stage (W0 = the iteration's start state, written in the first stage); integ 1:
 // explicit Euler step (W0 = the persistent conservative state rho, rho u, rho v, rho w, rho E).
+// parity: the stage's ping-pong parity (the walk direction when t.alternate is set).
 void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const DevFree &q, int scheme,
                       int integ, double *W0, double alpha, int firstStage, int machDerived, double *rmsPartial,
-                      const StageTuning &t, hipStream_t st);
+                      int parity, const StageTuning &t, hipStream_t st);
 // deterministic sum of n doubles into out[0] (single workgroup, fixed order)
 void launchSum(const double *in, int n, double *out, hipStream_t st);
 // interior Mach number from the primitives (FlowField.cpp:157), for state read-back

@@ -607,12 +607,14 @@ __global__ void __launch_bounds__(kBlock) kSmooth2(const double *R0, const doubl
 
 // Workgroups are dispatched round-robin over the XCDs (each with its own L2). With xcds > 1 the
 // dispatch index is remapped so that every XCD runs a contiguous range of blocks: neighbouring
-// blocks, which share cut faces and read each other's cells, then share an L2.
-__device__ __forceinline__ int logicalBlock(int w, int nBlocks, int xcds) {
-	if (xcds <= 1) return w;
+// blocks, which share cut faces and read each other's cells, then share an L2. With reverse set,
+// each XCD walks its range from the end.
+__device__ __forceinline__ int logicalBlock(int w, int nBlocks, int xcds, bool reverse) {
+	if (xcds <= 1) return reverse ? nBlocks - 1 - w : w;
 	const int x = w % xcds, k = w / xcds;
 	const int q = nBlocks / xcds, r = nBlocks % xcds;
-	return x * q + min(x, r) + k;
+	const int len = q + (x < r ? 1 : 0); // dispatch indices w with w % xcds == x
+	return x * q + min(x, r) + (reverse ? len - 1 - k : k);
 }
 
 // Fused RK stage over one block of cells (see the file comment). Every global load of the block
@@ -623,9 +625,9 @@ __device__ __forceinline__ int logicalBlock(int w, int nBlocks, int xcds) {
 template<int SCHEME, bool NT, bool MK, int INTEG>
 __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out, DevFree q,
                                                       double *W0, double alpha, int firstStage, int machDerived,
-                                                      double *rmsPartial, int xcds) {
+                                                      double *rmsPartial, int xcds, int reverse) {
 	extern __shared__ double lds[];
-	const int b = logicalBlock(blockIdx.x, p.nBlocks, xcds);
+	const int b = logicalBlock(blockIdx.x, p.nBlocks, xcds, reverse != 0);
 	const int t = threadIdx.x;
 	const int c0 = b * kBlock;
 	const int nc = min(kBlock, m.nElem - c0);
@@ -633,17 +635,25 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 	const int h0 = p.blockHalo[b], nh = p.blockHalo[b + 1] - h0;
 	const int cap = kBlock + p.maxHalo;
 	const int ms = p.maxSlots;
-	double *ls = lds;           // [6][cap]: rho, u, v, w, p, H of own cells then halo cells
-	double *lf = lds + 6 * cap; // [5][ms]: F*S per slot
+	double *ls = lds;                 // [6][cap]: rho, u, v, w, p, H of own cells then halo cells
+	double *lem = lds + 6 * cap;      // [2][kBlock]: E, M of own cells
+	double *lf = lem + 2 * kBlock;    // [5][ms]: F*S per slot
 	const size_t n = (size_t) m.nElem;
 
 	// ---- issue every load of the block
 	const int e = c0 + t;
 	const bool own = t < nc;
-	double2 ru = make_double2(0, 0), vw = ru, pE = ru, HM = ru;
-	if (own) {
-		const double2 *cc = reinterpret_cast<const double2 *>(cellPtr(in, e));
-		ru = cc[0], vw = cc[1], pE = cc[2], HM = cc[3];
+	// own records: the block's consecutive 64 B records are read as consecutive 16 B chunks,
+	// thread t taking chunks t, t + 256, t + 512, t + 768, so that every wave instruction covers
+	// whole lines; chunk j holds field pair j & 3 (rho u | v w | p E | H M) of cell j >> 2
+	double2 oc4[4];
+	{
+		const double2 *blk = reinterpret_cast<const double2 *>(cellPtr(in, c0));
+#pragma unroll
+		for (int i = 0; i < 4; i++) {
+			const int j = i * kBlock + t;
+			oc4[i] = j < 4 * nc ? blk[j] : make_double2(0, 0);
+		}
 	}
 	double w0[5], vol = 0, sx = 0, sy = 0, sz = 0;
 	uint16_t code[6];
@@ -671,13 +681,20 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 		ls[4 * cap + l] = a2.x;
 		ls[5 * cap + l] = a3.x;
 	}
-	if (own) {
-		ls[0 * cap + t] = ru.x;
-		ls[1 * cap + t] = ru.y;
-		ls[2 * cap + t] = vw.x;
-		ls[3 * cap + t] = vw.y;
-		ls[4 * cap + t] = pE.x;
-		ls[5 * cap + t] = HM.x;
+	{
+		// pair k = t & 3 of every chunk this thread loaded: rho|v|p|H into rows 0|2|4|5, the
+		// second member into rows 1|3 (u, w) or the own-cell E|M rows
+		const int k = t & 3;
+		double *dA = ls + (k < 2 ? 2 * k : 2 + k) * cap;
+		double *dB = k < 2 ? ls + (2 * k + 1) * cap : lem + (k - 2) * kBlock;
+#pragma unroll
+		for (int i = 0; i < 4; i++) {
+			const int j = i * kBlock + t;
+			if (j < 4 * nc) {
+				dA[j >> 2] = oc4[i].x;
+				dB[j >> 2] = oc4[i].y;
+			}
+		}
 	}
 	// face slot record of this thread's first round (later rounds are prefetched one ahead)
 	uint32_t pkN = 0;
@@ -717,9 +734,8 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 			const int c = p.e1[s];    // its ghost
 			Ghost G;
 			if (kind == 3) {
-				const double *ac = cellPtr(in, a);
-				double Mi = machDerived ? machOf(L.rho, L.u, L.v, L.w, L.p, q.gamma) : ac[S_M];
-				G = farfieldGhost(in, c, Mi, nx, ny, nz, q, L, ac[S_E]);
+				double Mi = machDerived ? machOf(L.rho, L.u, L.v, L.w, L.p, q.gamma) : lem[kBlock + l0];
+				G = farfieldGhost(in, c, Mi, nx, ny, nz, q, L, lem[l0]);
 			} else {
 				G = mirrorGhost(L.rho, L.u, L.v, L.w, L.p, nx, ny, nz, q.gamma);
 			}
@@ -766,14 +782,14 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 				for (int k = 0; k < 5; k++) r[k] += lf[k * ms + j];
 			}
 		}
-		const double rho = ru.x, u = ru.y, v = vw.x, w = vw.y, pp = pE.x;
+		const double rho = ls[t], u = ls[cap + t], v = ls[2 * cap + t], w = ls[3 * cap + t], pp = ls[4 * cap + t];
 		if (INTEG == 0 && firstStage) {
 			// W0 = (rho, u*rho, v*rho, w*rho, E*rho), EulerSolver.cpp:431-438
 			w0[0] = rho;
 			w0[1] = u * rho;
 			w0[2] = v * rho;
 			w0[3] = w * rho;
-			w0[4] = pE.y * rho;
+			w0[4] = lem[t] * rho;
 #pragma unroll
 			for (int k = 0; k < 5; k++) W0[k * n + e] = w0[k];
 		}
@@ -817,7 +833,9 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 		oc[0] = make_double2(nr, nu);
 		oc[1] = make_double2(nv, nw);
 		oc[2] = make_double2(np, nE);
-		oc[3].x = nH;
+		// the whole 64 B record is written (the stored Mach number carried over), so no line is
+		// left partly written
+		oc[3] = make_double2(nH, lem[kBlock + t]);
 		r2 = r[0] * r[0];
 	}
 	if (rmsPartial) {
@@ -885,6 +903,11 @@ inline int blocksFor(int n) { return (n + kBlock - 1) / kBlock; }
 
 int stageBlocks(int nElem) { return blocksFor(nElem); }
 
+size_t stageFusedLdsBytes(int maxSlots, int maxHalo) {
+	const size_t words = (size_t) 6 * (kBlock + maxHalo) + (size_t) 2 * kBlock + (size_t) 5 * maxSlots;
+	return sizeof(double) * words;
+}
+
 void launchBc(const DevMesh &m, DevState s, const DevFree &q, int machDerived, hipStream_t st) {
 	if (m.nGhost == 0) return;
 	hipLaunchKernelGGL(kBc, dim3(blocksFor(m.nGhost)), dim3(kBlock), 0, st, m, s, q, machDerived);
@@ -930,19 +953,19 @@ void launchSmooth(const double *R0, double *R1, double *R2, const int *nbr, cons
 
 void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const DevFree &q, int scheme,
                       int integ, double *W0, double alpha, int firstStage, int machDerived, double *rmsPartial,
-                      const StageTuning &t, hipStream_t st) {
+                      int parity, const StageTuning &t, hipStream_t st) {
 	if (p.nBlocks == 0) return;
-	size_t words = (size_t) 6 * (kBlock + p.maxHalo) + (size_t) 5 * p.maxSlots;
-	size_t lds = sizeof(double) * (words > (size_t) kBlock ? words : (size_t) kBlock);
+	const int rev = t.alternate ? (parity & 1) : 0;
+	const size_t lds = stageFusedLdsBytes(p.maxSlots, p.maxHalo);
 	const int x = t.xcds;
 #define E3D_LAUNCH(S, N, K)                                                                                          \
 	do {                                                                                                             \
 		if (integ == 1)                                                                                              \
 			hipLaunchKernelGGL((kStageFused<S, N, K, 1>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, q, \
-			                   W0, alpha, 0, machDerived, rmsPartial, x);                                          \
+			                   W0, alpha, 0, machDerived, rmsPartial, x, rev);                                     \
 		else                                                                                                         \
 			hipLaunchKernelGGL((kStageFused<S, N, K, 0>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, q, \
-			                   W0, alpha, firstStage, machDerived, rmsPartial, x);                                 \
+			                   W0, alpha, firstStage, machDerived, rmsPartial, x, rev);                            \
 	} while (0)
 #define E3D_LAUNCH_K(S, N)               \
 	if (t.sharedRecip) E3D_LAUNCH(S, N, true); \
