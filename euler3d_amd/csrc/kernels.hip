@@ -667,8 +667,16 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 		sy = ldStream<NT>(m.c.sy + e);
 		sz = ldStream<NT>(m.c.sz + e);
 		const uint16_t *cl = p.csr + (size_t) e * p.csrStride;
+		if (p.csrStride == 4) {
+			// tetrahedra: the cell's four codes as one 8 B load (rows are 8 B aligned)
+			const uint64_t c4 = ldStream<NT>(reinterpret_cast<const uint64_t *>(cl));
 #pragma unroll
-		for (int i = 0; i < 6; i++) code[i] = i < p.csrStride ? ldStream<NT>(cl + i) : (uint16_t) 0xFFFF;
+			for (int i = 0; i < 4; i++) code[i] = (uint16_t) (c4 >> (16 * i));
+			code[4] = code[5] = 0xFFFF;
+		} else {
+#pragma unroll
+			for (int i = 0; i < 6; i++) code[i] = i < p.csrStride ? ldStream<NT>(cl + i) : (uint16_t) 0xFFFF;
+		}
 	}
 	for (int h = t; h < nh; h += kBlock) {
 		const double2 *hc = reinterpret_cast<const double2 *>(cellPtr(in, p.haloCell[h0 + h]));

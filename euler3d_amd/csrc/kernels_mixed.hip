@@ -354,8 +354,16 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 		if (own) {
 			uint16_t code[6];
 			const uint16_t *cl = C.csr + (size_t) e * C.csrStride;
+			if (C.csrStride == 4) {
+				// tetrahedra: the four codes as one 8 B load (rows are 8 B aligned)
+				const uint64_t c4 = ldS<NT>(reinterpret_cast<const uint64_t *>(cl));
 #pragma unroll
-			for (int i = 0; i < 6; i++) code[i] = i < C.csrStride ? ldS<NT>(cl + i) : (uint16_t) 0xFFFF;
+				for (int i = 0; i < 4; i++) code[i] = (uint16_t) (c4 >> (16 * i));
+				code[4] = code[5] = 0xFFFF;
+			} else {
+#pragma unroll
+				for (int i = 0; i < 6; i++) code[i] = i < C.csrStride ? ldS<NT>(cl + i) : (uint16_t) 0xFFFF;
+			}
 			const D Lc = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y};
 			double r[5], cmp[5];
 #pragma unroll
