@@ -84,6 +84,10 @@ struct e3d_ctx {
 	// state: two buffers of (nElem + nGhost) x 8 doubles, st[cur] is the current one
 	DBuf<double> st[2];
 	int cur = 0;
+	// interior state as conservative W (5 doubles per cell) in wst[cur] instead of primitive
+	// records in st[cur]: the fused RK stage keeps it so, readers of primitives convert first
+	DBuf<double> wst[2];
+	bool wMode = false;
 	DBuf<double> W0, fs, partial, scalars;
 	// residual smoothing (RESIDUAL_SMOOTHING= 1): neighbour lists in device numbering, residual
 	// before/during/after the sweeps, per-block largest change, decision and NaN flags
@@ -106,6 +110,14 @@ struct e3d_ctx {
 
 	e3d::DevState state(int which) { return {st[which].p}; }
 	e3d::DevState state() { return state(cur); }
+	// primitive records of the interior cells in st[cur] (from wst[cur] when the RK stages left
+	// the state there)
+	void ensurePrims() {
+		if (!wMode) return;
+		e3d::launchWToPrim(dm, wst[cur].p, state(), fr, stream);
+		check();
+		wMode = false;
+	}
 	void sync() { hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
 	void check() { hipCheck(hipGetLastError(), "kernel launch"); }
 };
@@ -129,6 +141,7 @@ void initFreestream(e3d_ctx *c) {
 	c->W0.upload(w.data(), w.size(), c->stream);
 	c->sync();
 	c->machDerived = 0;
+	c->wMode = false;
 }
 
 void runStage(e3d_ctx *c, int stage, double *rmsPartial) {
@@ -152,15 +165,22 @@ void runStage(e3d_ctx *c, int stage, double *rmsPartial) {
 	const int ti = c->prm.time_integration;
 	const bool smooth = c->prm.residual_smoothing != 0;
 	if (c->fused && !smooth && ti != 1) {
-		// one fused launch: ghosts, fluxes, gather, dt, update (RK stage or explicit Euler step)
+		// one fused launch: ghosts, fluxes, gather, dt, update (RK stage or explicit Euler step);
+		// RK stages leave the interior state as W, the explicit Euler step as primitives
+		const int integ = ti == 0 ? 1 : 0;
+		if (integ == 1) c->ensurePrims();
+		const int layout = integ == 1 ? 0 : c->wMode ? 2 : 1;
 		auto o = c->state(c->cur ^ 1);
 		record(evA);
-		e3d::launchStageFused(c->dm, c->plan, s, o, c->fr, c->prm.scheme, ti == 0 ? 1 : 0, c->W0.p, alphas[stage],
-		                      stage == 0, c->machDerived, rmsPartial, c->cur, c->tune, c->stream);
+		e3d::launchStageFused(c->dm, c->plan, s, o, c->wst[c->cur].p, c->wst[c->cur ^ 1].p, layout, c->fr,
+		                      c->prm.scheme, integ, c->W0.p, alphas[stage], stage == 0, c->machDerived, rmsPartial,
+		                      c->cur, c->tune, c->stream);
 		record(evB);
 		c->cur ^= 1;
+		c->wMode = layout != 0;
 	} else {
 		// EulerSolver::Run / RungeKutta body: updateBC, computeResidual, smoothResiduals, update in place
+		c->ensurePrims();
 		record(evA);
 		e3d::launchBc(c->dm, s, c->fr, c->machDerived, c->stream);
 		e3d::launchFlux(c->dm, s, c->fr, c->prm.scheme, c->fs.p, nullptr, c->stream);
@@ -685,6 +705,7 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		c->cpFactor = 1 / (0.5 * c->fr.rho_inf * uinf2);
 
 		for (auto &b : c->st) b.alloc((size_t) c->nTot * e3d::kStateWidth);
+		for (auto &b : c->wst) b.alloc((size_t) 5 * nE + 2);
 		c->W0.alloc(5 * nE);
 		if (p->residual_smoothing) {
 			// neighbour of each local face in device numbering (EulerSolver.cpp:495-503 walks
@@ -755,6 +776,7 @@ int e3d_set_state(e3d_ctx *c, const double *const fields[8]) {
 		c->W0.upload(w.data(), w.size(), c->stream);
 		c->sync();
 		c->machDerived = 0;
+		c->wMode = false;
 		return E3D_OK;
 	});
 }
@@ -763,6 +785,7 @@ int e3d_get_state(e3d_ctx *c, double *const fields[8]) {
 	if (!c || !fields) return fail(E3D_ERR_ARG, "null argument");
 	return guarded([&] {
 		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		c->ensurePrims();
 		if (c->machDerived) {
 			e3d::launchMach(c->dm, c->state(), c->fr, c->stream);
 			c->check();
@@ -786,6 +809,7 @@ int e3d_update_bc(e3d_ctx *c) {
 	if (!c) return fail(E3D_ERR_ARG, "null context");
 	return guarded([&] {
 		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		c->ensurePrims();
 		e3d::launchBc(c->dm, c->state(), c->fr, c->machDerived, c->stream);
 		c->check();
 		c->sync();
@@ -806,6 +830,7 @@ int e3d_face_flux(e3d_ctx *c, double *F, double *FS) {
 		hipCheck(hipSetDevice(c->device), "hipSetDevice");
 		DBuf<double> raw;
 		raw.alloc(5 * (size_t) c->nFace);
+		c->ensurePrims();
 		e3d::launchFlux(c->dm, c->state(), c->fr, c->prm.scheme, c->fs.p, raw.p, c->stream);
 		c->check();
 		std::vector<double> h(5 * (size_t) c->nFace);
@@ -830,6 +855,7 @@ int e3d_residual(e3d_ctx *c, double *R) {
 		hipCheck(hipSetDevice(c->device), "hipSetDevice");
 		DBuf<double> dR;
 		dR.alloc(5 * (size_t) c->nElem);
+		c->ensurePrims();
 		e3d::launchFlux(c->dm, c->state(), c->fr, c->prm.scheme, c->fs.p, nullptr, c->stream);
 		e3d::launchGather(c->dm, c->fs.p, dR.p, c->stream);
 		c->check();
@@ -850,6 +876,7 @@ int e3d_timestep(e3d_ctx *c, double *dt) {
 		hipCheck(hipSetDevice(c->device), "hipSetDevice");
 		DBuf<double> d;
 		d.alloc(c->nElem);
+		c->ensurePrims();
 		e3d::launchTimestep(c->dm, c->state(), c->fr, d.p, c->stream);
 		c->check();
 		std::vector<double> h(c->nElem);
@@ -894,7 +921,8 @@ int e3d_aero(e3d_ctx *c, double cf[3], double cm[3]) {
 			if (!c->nWallPart[r]) continue;
 			const int w0 = c->wall0[r];
 			e3d::launchAero(c->nWallPart[r], c->wallCell.p + w0, c->wN[0].p + w0, c->wN[1].p + w0, c->wN[2].p + w0,
-			                c->wC[0].p + w0, c->wC[1].p + w0, c->wC[2].p + w0, c->state(), c->cpFactor, c->fr.p_inf,
+			                c->wC[0].p + w0, c->wC[1].p + w0, c->wC[2].p + w0, c->state(),
+			                c->wMode ? c->wst[c->cur].p : nullptr, c->fr.gamma, c->cpFactor, c->fr.p_inf,
 			                c->prm.cref, c->prm.sref, c->aeroOut.p + 6 * (size_t) r, c->stream);
 			c->check();
 			any = true;
@@ -996,6 +1024,7 @@ int e3d_checkpoint_write(e3d_ctx *c, const char *path) {
 		hipCheck(hipSetDevice(c->device), "hipSetDevice");
 		const size_t nT = (size_t) c->nTot, nE = (size_t) c->nElem;
 		std::vector<double> st(nT * e3d::kStateWidth), w(5 * nE);
+		c->ensurePrims();
 		hipCheck(hipMemcpyAsync(st.data(), c->st[c->cur].p, st.size() * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
 		hipCheck(hipMemcpyAsync(w.data(), c->W0.p, w.size() * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
 		c->sync();
@@ -1052,6 +1081,7 @@ int e3d_checkpoint_read(e3d_ctx *c, const char *path) {
 		c->W0.upload(w.data(), w.size(), c->stream);
 		c->sync();
 		c->machDerived = h.machDerived;
+		c->wMode = false;
 		return E3D_OK;
 	});
 }

@@ -114,16 +114,26 @@ struct StageTuning {
 // integ 0: RK stage (W0 = the iteration's start state, written in the first stage); integ 1:
 // explicit Euler step (W0 = the persistent conservative state rho, rho u, rho v, rho w, rho E).
 // parity: the stage's ping-pong parity (the walk direction when t.alternate is set).
-void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const DevFree &q, int scheme,
-                      int integ, double *W0, double alpha, int firstStage, int machDerived, double *rmsPartial,
-                      int parity, const StageTuning &t, hipStream_t st);
+// layout (RK stages only; the explicit Euler step is always 0): where the interior state lives.
+//   0: 64 B primitive records in `in` -> `out`;
+//   1: primitive records in `in` -> conservative states (5 doubles per cell) in Wout;
+//   2: conservative states Win -> Wout.
+// Ghost records are read from `in` and written to `out` in every layout. The conservative buffers
+// hold 5 * nElem doubles plus two of padding.
+void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const double *Win, double *Wout,
+                      int layout, const DevFree &q, int scheme, int integ, double *W0, double alpha, int firstStage,
+                      int machDerived, double *rmsPartial, int parity, const StageTuning &t, hipStream_t st);
+// interior primitive records of s from the conservative states W (the stage's expressions), with
+// M = c / |u|
+void launchWToPrim(const DevMesh &m, const double *W, DevState s, const DevFree &q, hipStream_t st);
 // deterministic sum of n doubles into out[0] (single workgroup, fixed order)
 void launchSum(const double *in, int n, double *out, hipStream_t st);
 // interior Mach number from the primitives (FlowField.cpp:157), for state read-back
 void launchMach(const DevMesh &m, DevState s, const DevFree &q, hipStream_t st);
 // wall-face force/moment: out[6] = (Fx,Fy,Fz, CPx,CPy,CPz) summed in a fixed order
+// (wall cells' p from W when W is not null, else from the primitive records of s)
 void launchAero(int nWall, const int *wallCell, const double *Nx, const double *Ny, const double *Nz,
-                const double *Cx, const double *Cy, const double *Cz, DevState s, double cpFactor, double pInf,
-                double cref, double sref, double *out6, hipStream_t st);
+                const double *Cx, const double *Cy, const double *Cz, DevState s, const double *W, double gamma,
+                double cpFactor, double pInf, double cref, double sref, double *out6, hipStream_t st);
 
 }// namespace e3d

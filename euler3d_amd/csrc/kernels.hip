@@ -123,6 +123,21 @@ __device__ __forceinline__ double quot(double x, double d, double y) {
 	else return x / d;
 }
 
+// primitives of a conservative state as FlowField::updateWRungeKutta forms them
+// (FlowField.cpp:140-158): u = rho u / rho, ..., E = rho E / rho, p = (gamma - 1) rho
+// (E - |u|^2 / 2), H = E + p / rho
+template<bool MK>
+__device__ __forceinline__ void primOf(double r, double ru, double rv, double rw, double rE, double g, double &u,
+                                       double &v, double &w, double &pr, double &E, double &H) {
+	const double yr = MK ? 1.0 / r : 0.0;
+	u = quot<MK>(ru, r, yr);
+	v = quot<MK>(rv, r, yr);
+	w = quot<MK>(rw, r, yr);
+	E = quot<MK>(rE, r, yr);
+	pr = (g - 1) * r * (E - (u * u + v * v + w * w) * 0.5);
+	H = E + quot<MK>(pr, r, yr);
+}
+
 // The same Roe flux, component by component: each of the five values is formed with exactly the
 // operations of roeFlux and handed to emit(k, F_k) at once, so no 5-vector of temporaries is live.
 // With MK the quotients by sL + sR and by 2 c~^2 go through divRecip.
@@ -622,9 +637,10 @@ __device__ __forceinline__ int logicalBlock(int w, int nBlocks, int xcds, bool r
 // and its halo (the outside cells its faces read) are staged in LDS as rho, u, v, w, p, H; the
 // update's inputs (W0, V, sum|S|, face slots) are loaded into registers. The flux loop then reads
 // LDS only, except for boundary faces, whose ghost cells live in global memory.
-template<int SCHEME, bool NT, bool MK, int INTEG>
-__global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out, DevFree q,
-                                                      double *W0, double alpha, int firstStage, int machDerived,
+template<int SCHEME, bool NT, bool MK, int INTEG, int LAYOUT>
+__global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out,
+                                                      const double *Win, double *Wout, DevFree q, double *W0,
+                                                      double alpha, int firstStage, int machDerived,
                                                       double *rmsPartial, int xcds, int reverse) {
 	extern __shared__ double lds[];
 	const int b = logicalBlock(blockIdx.x, p.nBlocks, xcds, reverse != 0);
@@ -637,7 +653,7 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 	const int ms = p.maxSlots;
 	double *ls = lds;                 // [6][cap]: rho, u, v, w, p, H of own cells then halo cells
 	double *lem = lds + 6 * cap;      // [2][kBlock]: E, M of own cells
-	double *lf = lem + 2 * kBlock;    // [5][ms]: F*S per slot
+	double *lf = lem + 2 * kBlock;    // [5][ms]: F*S per slot (first the own W of LAYOUT 2)
 	const size_t n = (size_t) m.nElem;
 
 	// ---- issue every load of the block
@@ -647,13 +663,24 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 	// thread t taking chunks t, t + 256, t + 512, t + 768, so that every wave instruction covers
 	// whole lines; chunk j holds field pair j & 3 (rho u | v w | p E | H M) of cell j >> 2
 	double2 oc4[4];
-	{
+	if constexpr (LAYOUT < 2) {
 		const double2 *blk = reinterpret_cast<const double2 *>(cellPtr(in, c0));
 #pragma unroll
 		for (int i = 0; i < 4; i++) {
 			const int j = i * kBlock + t;
 			oc4[i] = j < 4 * nc ? blk[j] : make_double2(0, 0);
 		}
+	} else {
+		// own conservative states: 5 doubles per cell, the block's range (16 B aligned) read as
+		// consecutive 16 B chunks; chunk j holds doubles 2j and 2j + 1 of the range (the last
+		// chunk of the last block may reach one double into the buffer's padding)
+		const double2 *blk = reinterpret_cast<const double2 *>(Win + (size_t) 5 * c0);
+#pragma unroll
+		for (int i = 0; i < 3; i++) {
+			const int j = i * kBlock + t;
+			oc4[i] = 2 * j < 5 * nc ? blk[j] : make_double2(0, 0);
+		}
+		oc4[3] = make_double2(0, 0);
 	}
 	double w0[5], vol = 0, sx = 0, sy = 0, sz = 0;
 	uint16_t code[6];
@@ -679,17 +706,40 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 		}
 	}
 	for (int h = t; h < nh; h += kBlock) {
-		const double2 *hc = reinterpret_cast<const double2 *>(cellPtr(in, p.haloCell[h0 + h]));
-		const double2 a0 = hc[0], a1 = hc[1], a2 = hc[2], a3 = hc[3];
 		const int l = kBlock + h;
-		ls[0 * cap + l] = a0.x;
-		ls[1 * cap + l] = a0.y;
-		ls[2 * cap + l] = a1.x;
-		ls[3 * cap + l] = a1.y;
-		ls[4 * cap + l] = a2.x;
-		ls[5 * cap + l] = a3.x;
+		if constexpr (LAYOUT < 2) {
+			const double2 *hc = reinterpret_cast<const double2 *>(cellPtr(in, p.haloCell[h0 + h]));
+			const double2 a0 = hc[0], a1 = hc[1], a2 = hc[2], a3 = hc[3];
+			ls[0 * cap + l] = a0.x;
+			ls[1 * cap + l] = a0.y;
+			ls[2 * cap + l] = a1.x;
+			ls[3 * cap + l] = a1.y;
+			ls[4 * cap + l] = a2.x;
+			ls[5 * cap + l] = a3.x;
+		} else {
+			const double *hw = Win + (size_t) 5 * p.haloCell[h0 + h];
+			const double r = hw[0];
+			double u, v, w, pr, E, H;
+			primOf<MK>(r, hw[1], hw[2], hw[3], hw[4], q.gamma, u, v, w, pr, E, H);
+			ls[0 * cap + l] = r;
+			ls[1 * cap + l] = u;
+			ls[2 * cap + l] = v;
+			ls[3 * cap + l] = w;
+			ls[4 * cap + l] = pr;
+			ls[5 * cap + l] = H;
+		}
 	}
-	{
+	if constexpr (LAYOUT == 2) {
+		// the own W range as loaded, into LDS; turned into primitives after the barrier
+#pragma unroll
+		for (int i = 0; i < 3; i++) {
+			const int j = i * kBlock + t;
+			if (2 * j < 5 * nc) {
+				lf[2 * j] = oc4[i].x;
+				lf[2 * j + 1] = oc4[i].y;
+			}
+		}
+	} else {
 		// pair k = t & 3 of every chunk this thread loaded: rho|v|p|H into rows 0|2|4|5, the
 		// second member into rows 1|3 (u, w) or the own-cell E|M rows
 		const int k = t & 3;
@@ -713,6 +763,22 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 		SN = ldStream<NT>(p.S + s0 + t);
 	}
 	__syncthreads();
+	if constexpr (LAYOUT == 2) {
+		if (own) {
+			const double *wt = lf + 5 * t;
+			const double r = wt[0];
+			double u, v, w, pr, E, H;
+			primOf<MK>(r, wt[1], wt[2], wt[3], wt[4], q.gamma, u, v, w, pr, E, H);
+			ls[t] = r;
+			ls[cap + t] = u;
+			ls[2 * cap + t] = v;
+			ls[3 * cap + t] = w;
+			ls[4 * cap + t] = pr;
+			ls[5 * cap + t] = H;
+			lem[t] = E;
+		}
+		__syncthreads();
+	}
 
 	// ---- face fluxes of the block, once each, into LDS
 	auto lprim = [&](int l) -> Prim {
@@ -831,19 +897,26 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 			nrw = w0[3] - quot<MK>(alpha * dt * r[3], vol, yv);
 			nrE = w0[4] - quot<MK>(alpha * dt * r[4], vol, yv);
 		}
-		const double yr = MK ? 1.0 / nr : 0.0;
-		double nu = quot<MK>(nru, nr, yr), nv = quot<MK>(nrv, nr, yr), nw = quot<MK>(nrw, nr, yr);
-		double nE = quot<MK>(nrE, nr, yr);
-		double g = q.gamma;
-		double np = (g - 1) * nr * (nE - (nu * nu + nv * nv + nw * nw) * 0.5);
-		double nH = nE + quot<MK>(np, nr, yr);
-		double2 *oc = reinterpret_cast<double2 *>(cellPtr(out, e));
-		oc[0] = make_double2(nr, nu);
-		oc[1] = make_double2(nv, nw);
-		oc[2] = make_double2(np, nE);
-		// the whole 64 B record is written (the stored Mach number carried over), so no line is
-		// left partly written
-		oc[3] = make_double2(nH, lem[kBlock + t]);
+		if constexpr (LAYOUT == 0) {
+			double nu, nv, nw, np, nE, nH;
+			primOf<MK>(nr, nru, nrv, nrw, nrE, q.gamma, nu, nv, nw, np, nE, nH);
+			double2 *oc = reinterpret_cast<double2 *>(cellPtr(out, e));
+			oc[0] = make_double2(nr, nu);
+			oc[1] = make_double2(nv, nw);
+			oc[2] = make_double2(np, nE);
+			// the whole 64 B record is written (the stored Mach number carried over), so no line
+			// is left partly written
+			oc[3] = make_double2(nH, lem[kBlock + t]);
+		} else {
+			// the conservative state only: the next stage (or the read-back) forms the primitives
+			// from it with the same expressions
+			double *ow = Wout + (size_t) 5 * e;
+			ow[0] = nr;
+			ow[1] = nru;
+			ow[2] = nrv;
+			ow[3] = nrw;
+			ow[4] = nrE;
+		}
 		r2 = r[0] * r[0];
 	}
 	if (rmsPartial) {
@@ -851,6 +924,21 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 		double tt = blockSum(r2, lds);
 		if (t == 0) rmsPartial[b] = tt;
 	}
+}
+
+// interior primitives (64 B records) from the conservative states the fused RK stage keeps
+__global__ void __launch_bounds__(kBlock) kWToPrim(DevMesh m, const double *W, DevState s, DevFree q) {
+	const int e = blockIdx.x * kBlock + threadIdx.x;
+	if (e >= m.nElem) return;
+	const double *w5 = W + (size_t) 5 * e;
+	const double r = w5[0];
+	double u, v, w, pr, E, H;
+	primOf<false>(r, w5[1], w5[2], w5[3], w5[4], q.gamma, u, v, w, pr, E, H);
+	double2 *c = reinterpret_cast<double2 *>(cellPtr(s, e));
+	c[0] = make_double2(r, u);
+	c[1] = make_double2(v, w);
+	c[2] = make_double2(pr, E);
+	c[3] = make_double2(H, machOf(r, u, v, w, pr, q.gamma));
 }
 
 // deterministic sum of n doubles: each of kSumThreads threads adds the strided elements
@@ -880,14 +968,23 @@ __global__ void __launch_bounds__(kBlock) kMach(DevMesh m, DevState s, DevFree q
 // coeff*x_face; summed per thread over a contiguous chunk, then a fixed tree
 __global__ void __launch_bounds__(kBlock) kAero(int nWall, const int *wallCell, const double *Nx, const double *Ny,
                                                 const double *Nz, const double *Cx, const double *Cy, const double *Cz,
-                                                DevState s, double cpFactor, double pInf, double cref, double sref,
-                                                double *out6) {
+                                                DevState s, const double *W, double gamma, double cpFactor, double pInf,
+                                                double cref, double sref, double *out6) {
 	__shared__ double lds[kBlock];
 	double acc[6] = {0, 0, 0, 0, 0, 0};
 	int per = (nWall + kBlock - 1) / kBlock;
 	int b = threadIdx.x * per, en = min(nWall, b + per);
 	for (int i = b; i < en; i++) {
-		double cp = cpFactor * (cellPtr(s, wallCell[i])[S_P] - pInf);
+		double pw;
+		if (W) {
+			// p of the conservative state, as the update forms it
+			const double *w5 = W + (size_t) 5 * wallCell[i];
+			double u, v, w, E, H;
+			primOf<false>(w5[0], w5[1], w5[2], w5[3], w5[4], gamma, u, v, w, pw, E, H);
+		} else {
+			pw = cellPtr(s, wallCell[i])[S_P];
+		}
+		double cp = cpFactor * (pw - pInf);
 		double cx = Nx[i] * cp * cref / sref;
 		double cy = Ny[i] * cp * cref / sref;
 		double cz = Nz[i] * cp * cref / sref;
@@ -912,8 +1009,15 @@ inline int blocksFor(int n) { return (n + kBlock - 1) / kBlock; }
 int stageBlocks(int nElem) { return blocksFor(nElem); }
 
 size_t stageFusedLdsBytes(int maxSlots, int maxHalo) {
-	const size_t words = (size_t) 6 * (kBlock + maxHalo) + (size_t) 2 * kBlock + (size_t) 5 * maxSlots;
+	// the flux region also stages the block's conservative states (5 per cell) before the fluxes
+	const size_t flux = (size_t) 5 * (maxSlots > kBlock ? maxSlots : kBlock);
+	const size_t words = (size_t) 6 * (kBlock + maxHalo) + (size_t) 2 * kBlock + flux;
 	return sizeof(double) * words;
+}
+
+void launchWToPrim(const DevMesh &m, const double *W, DevState s, const DevFree &q, hipStream_t st) {
+	if (m.nElem == 0) return;
+	hipLaunchKernelGGL(kWToPrim, dim3(blocksFor(m.nElem)), dim3(kBlock), 0, st, m, W, s, q);
 }
 
 void launchBc(const DevMesh &m, DevState s, const DevFree &q, int machDerived, hipStream_t st) {
@@ -959,21 +1063,24 @@ void launchSmooth(const double *R0, double *R1, double *R2, const int *nbr, cons
 	                   nPart);
 }
 
-void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const DevFree &q, int scheme,
-                      int integ, double *W0, double alpha, int firstStage, int machDerived, double *rmsPartial,
-                      int parity, const StageTuning &t, hipStream_t st) {
+void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const double *Win, double *Wout,
+                      int layout, const DevFree &q, int scheme, int integ, double *W0, double alpha, int firstStage,
+                      int machDerived, double *rmsPartial, int parity, const StageTuning &t, hipStream_t st) {
 	if (p.nBlocks == 0) return;
 	const int rev = t.alternate ? (parity & 1) : 0;
 	const size_t lds = stageFusedLdsBytes(p.maxSlots, p.maxHalo);
 	const int x = t.xcds;
-#define E3D_LAUNCH(S, N, K)                                                                                          \
-	do {                                                                                                             \
-		if (integ == 1)                                                                                              \
-			hipLaunchKernelGGL((kStageFused<S, N, K, 1>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, q, \
-			                   W0, alpha, 0, machDerived, rmsPartial, x, rev);                                     \
-		else                                                                                                         \
-			hipLaunchKernelGGL((kStageFused<S, N, K, 0>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, q, \
-			                   W0, alpha, firstStage, machDerived, rmsPartial, x, rev);                            \
+#define E3D_LAUNCH(S, N, K)                                                                                    \
+	do {                                                                                                       \
+		if (integ == 1)                                                                                        \
+			hipLaunchKernelGGL((kStageFused<S, N, K, 1, 0>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, \
+			                   Win, Wout, q, W0, alpha, 0, machDerived, rmsPartial, x, rev);                      \
+		else if (layout == 2)                                                                                  \
+			hipLaunchKernelGGL((kStageFused<S, N, K, 0, 2>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, \
+			                   Win, Wout, q, W0, alpha, firstStage, machDerived, rmsPartial, x, rev);             \
+		else                                                                                                   \
+			hipLaunchKernelGGL((kStageFused<S, N, K, 0, 1>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, \
+			                   Win, Wout, q, W0, alpha, firstStage, machDerived, rmsPartial, x, rev);             \
 	} while (0)
 #define E3D_LAUNCH_K(S, N)               \
 	if (t.sharedRecip) E3D_LAUNCH(S, N, true); \
@@ -999,10 +1106,10 @@ void launchMach(const DevMesh &m, DevState s, const DevFree &q, hipStream_t st) 
 }
 
 void launchAero(int nWall, const int *wallCell, const double *Nx, const double *Ny, const double *Nz, const double *Cx,
-                const double *Cy, const double *Cz, DevState s, double cpFactor, double pInf, double cref, double sref,
-                double *out6, hipStream_t st) {
-	hipLaunchKernelGGL(kAero, dim3(1), dim3(kBlock), 0, st, nWall, wallCell, Nx, Ny, Nz, Cx, Cy, Cz, s, cpFactor, pInf,
-	                   cref, sref, out6);
+                const double *Cy, const double *Cz, DevState s, const double *W, double gamma, double cpFactor,
+                double pInf, double cref, double sref, double *out6, hipStream_t st) {
+	hipLaunchKernelGGL(kAero, dim3(1), dim3(kBlock), 0, st, nWall, wallCell, Nx, Ny, Nz, Cx, Cy, Cz, s, W, gamma,
+	                   cpFactor, pInf, cref, sref, out6);
 }
 
 }// namespace e3d
