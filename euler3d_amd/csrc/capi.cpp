@@ -646,7 +646,8 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		if (const char *v = std::getenv("E3D_SHARED_RECIP")) c->tune.sharedRecip = std::atoi(v) != 0;
 		if (const char *v = std::getenv("E3D_ALTERNATE")) c->tune.alternate = std::atoi(v) != 0;
 		e3d::HostPlan P = e3d::buildPlan(sm, order, e3d::kCellsPerBlock);
-		if (e3d::stageFusedLdsBytes(P.maxSlots, P.maxHalo) > 160 * 1024)
+		if (std::max(e3d::stageFusedLdsBytes(P.maxSlots, P.maxHalo, 1), e3d::stageFusedLdsBytes(P.maxSlots, P.maxHalo, 2)) >
+		    160 * 1024)
 			return fail(E3D_ERR_UNSUPPORTED, "a block of the fused stage needs more LDS than a CU has");
 		c->new2old = P.new2old;
 

@@ -94,8 +94,8 @@ void launchStage(const DevMesh &m, DevState s, const DevFree &q, const double *f
 void launchSmooth(const double *R0, double *R1, double *R2, const int *nbr, const uint8_t *nface, int stride, int nElem,
                   double *blockMax, int *flag, const int *partStart, int nPart, double *cellMax, hipStream_t st);
 int stageBlocks(int nElem);
-// dynamic LDS of one fused-stage workgroup for a plan's largest block
-size_t stageFusedLdsBytes(int maxSlots, int maxHalo);
+// dynamic LDS of one fused-stage workgroup for a plan's largest block, per state layout
+size_t stageFusedLdsBytes(int maxSlots, int maxHalo, int layout);
 // launch choices of the fused stage (they change speed only, never results)
 struct StageTuning {
 	bool nontemporal = true; // non-temporal loads for arrays read once per stage

@@ -25,6 +25,9 @@ namespace e3d {
 namespace {
 
 constexpr int kBlock = 256;
+// the per-cell value the fused stage keeps in LDS for the Roe flux with conservative states in:
+// 0 sqrt(rho) (two square roots less per face), 1 the sound speed (a division and a root less)
+constexpr int kRoeRow = 0;
 static_assert(kBlock == kCellsPerBlock, "one thread per cell in the fused stage");
 
 struct Prim {
@@ -140,11 +143,11 @@ __device__ __forceinline__ void primOf(double r, double ru, double rv, double rw
 
 // The same Roe flux, component by component: each of the five values is formed with exactly the
 // operations of roeFlux and handed to emit(k, F_k) at once, so no 5-vector of temporaries is live.
-// With MK the quotients by sL + sR and by 2 c~^2 go through divRecip.
+// With MK the quotients by sL + sR and by 2 c~^2 go through divRecip. The per-cell values
+// sL = sqrt(rho_L), sR = sqrt(rho_R) and cL = sqrt(gamma p_L / rho_L) come from the caller.
 template<bool MK = false, typename Emit>
-__device__ __forceinline__ void roeFluxEmit(const Prim &L, const Prim &R, double nx, double ny, double nz, double gamma,
-                                            Emit emit) {
-	double sL = sqrt(L.rho), sR = sqrt(R.rho);
+__device__ __forceinline__ void roeFluxEmit(const Prim &L, const Prim &R, double sL, double sR, double cL, double nx,
+                                            double ny, double nz, double gamma, Emit emit) {
 	double rhoT = sqrt(L.rho * R.rho);
 	const double den = sL + sR;
 	const double yden = MK ? 1.0 / den : 0.0;
@@ -158,8 +161,7 @@ __device__ __forceinline__ void roeFluxEmit(const Prim &L, const Prim &R, double
 	double VL = L.u * nx + L.v * ny + L.w * nz;
 	double VR = R.u * nx + R.v * ny + R.w * nz;
 	double dP = R.p - L.p, dVc = VR - VL;
-	double localC = sqrt(gamma * L.p / L.rho);
-	double delta = (1.0 / 15.0) * localC;
+	double delta = (1.0 / 15.0) * cL;
 	double a1 = VT - cT, a5 = VT + cT;
 	// Harten's fix on the acoustic waves; the quotients are formed only where a lane needs them
 	double l1 = fabs(a1), l5 = fabs(a5);
@@ -204,9 +206,9 @@ __device__ __forceinline__ double presMinus(double M) {
 	return fabs(M) <= 1.0 ? 0.25 * (M - 1.0) * (M - 1.0) * (2.0 + M) : (M < 0.0 ? 1.0 : 0.0);
 }
 
-__device__ __forceinline__ void ausmFlux(const Prim &L, const Prim &R, double nx, double ny, double nz, double g,
-                                         double out[5]) {
-	double aL = sqrt(g * L.p / L.rho), aR = sqrt(g * R.p / R.rho);
+// with the sound speeds aL = sqrt(g p_L / rho_L), aR = sqrt(g p_R / rho_R) from the caller
+__device__ __forceinline__ void ausmFluxA(const Prim &L, const Prim &R, double aL, double aR, double nx, double ny,
+                                          double nz, double g, double out[5]) {
 	double VL = L.u * nx + L.v * ny + L.w * nz;
 	double VR = R.u * nx + R.v * ny + R.w * nz;
 	double ML = VL / aL, MR = VR / aR;
@@ -217,6 +219,11 @@ __device__ __forceinline__ void ausmFlux(const Prim &L, const Prim &R, double nx
 	double pn[5] = {0.0, ph * nx, ph * ny, ph * nz, 0.0};
 #pragma unroll
 	for (int i = 0; i < 5; i++) out[i] = mh * 0.5 * (phiL[i] + phiR[i]) - 0.5 * fabs(mh) * (phiR[i] - phiL[i]) + pn[i];
+}
+
+__device__ __forceinline__ void ausmFlux(const Prim &L, const Prim &R, double nx, double ny, double nz, double g,
+                                         double out[5]) {
+	ausmFluxA(L, R, sqrt(g * L.p / L.rho), sqrt(g * R.p / R.rho), nx, ny, nz, g, out);
 }
 
 template<int SCHEME>
@@ -651,9 +658,17 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 	const int h0 = p.blockHalo[b], nh = p.blockHalo[b + 1] - h0;
 	const int cap = kBlock + p.maxHalo;
 	const int ms = p.maxSlots;
-	double *ls = lds;                 // [6][cap]: rho, u, v, w, p, H of own cells then halo cells
-	double *lem = lds + 6 * cap;      // [2][kBlock]: E, M of own cells
-	double *lf = lem + 2 * kBlock;    // [5][ms]: F*S per slot (first the own W of LAYOUT 2)
+	// LDS: [6][cap] rho, u, v, w, p, H of own cells then halo cells; with LAYOUT 2 a row [cap] of
+	// one per-cell value of the flux (cellRow) and [kBlock] E of own cells, else [2][kBlock] E, M
+	// of own cells; then [5][ms] F*S per slot (first the own W of LAYOUT 2)
+	double *ls = lds;
+	double *lcr = ls + 6 * cap;
+	double *lem = LAYOUT == 2 ? lcr + cap : ls + 6 * cap;
+	double *lf = LAYOUT == 2 ? lem + kBlock : lem + 2 * kBlock;
+	// Roe: sqrt(rho), AUSM: the sound speed sqrt(gamma p / rho), as the flux forms them
+	auto cellRow = [&](double r, double pr) -> double {
+		return (SCHEME == 0 && kRoeRow == 0) ? sqrt(r) : sqrt(q.gamma * pr / r);
+	};
 	const size_t n = (size_t) m.nElem;
 
 	// ---- issue every load of the block
@@ -727,6 +742,7 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 			ls[3 * cap + l] = w;
 			ls[4 * cap + l] = pr;
 			ls[5 * cap + l] = H;
+			lcr[l] = cellRow(r, pr);
 		}
 	}
 	if constexpr (LAYOUT == 2) {
@@ -775,6 +791,7 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 			ls[3 * cap + t] = w;
 			ls[4 * cap + t] = pr;
 			ls[5 * cap + t] = H;
+			lcr[t] = cellRow(r, pr);
 			lem[t] = E;
 		}
 		__syncthreads();
@@ -797,18 +814,28 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 		const Prim L = lprim(l0);
 		auto emit = [&](int k, double f) { lf[k * ms + j] = f * S; };
 		if (SCHEME == 0 && kind == 0) {
-			roeFluxEmit<MK>(L, lprim(l1), nx, ny, nz, q.gamma, emit);
+			const Prim R = lprim(l1);
+			double sL, sR, cL;
+			if constexpr (LAYOUT == 2 && kRoeRow == 0) {
+				sL = lcr[l0], sR = lcr[l1], cL = sqrt(q.gamma * L.p / L.rho);
+			} else if constexpr (LAYOUT == 2) {
+				sL = sqrt(L.rho), sR = sqrt(R.rho), cL = lcr[l0];
+			} else {
+				sL = sqrt(L.rho), sR = sqrt(R.rho), cL = sqrt(q.gamma * L.p / L.rho);
+			}
+			roeFluxEmit<MK>(L, R, sL, sR, cL, nx, ny, nz, q.gamma, emit);
 			continue;
 		}
 		double F[5];
 		if (kind == 0) {
-			schemeFlux<SCHEME>(L, lprim(l1), nx, ny, nz, q.gamma, F);
+			if constexpr (SCHEME == 1 && LAYOUT == 2) ausmFluxA(L, lprim(l1), lcr[l0], lcr[l1], nx, ny, nz, q.gamma, F);
+			else schemeFlux<SCHEME>(L, lprim(l1), nx, ny, nz, q.gamma, F);
 		} else {
 			const int a = c0 + l0;    // the interior cell of a boundary face is an own cell
 			const int c = p.e1[s];    // its ghost
 			Ghost G;
 			if (kind == 3) {
-				double Mi = machDerived ? machOf(L.rho, L.u, L.v, L.w, L.p, q.gamma) : lem[kBlock + l0];
+				double Mi = (LAYOUT == 2 || machDerived) ? machOf(L.rho, L.u, L.v, L.w, L.p, q.gamma) : lem[kBlock + l0];
 				G = farfieldGhost(in, c, Mi, nx, ny, nz, q, L, lem[l0]);
 			} else {
 				G = mirrorGhost(L.rho, L.u, L.v, L.w, L.p, nx, ny, nz, q.gamma);
@@ -1008,10 +1035,11 @@ inline int blocksFor(int n) { return (n + kBlock - 1) / kBlock; }
 
 int stageBlocks(int nElem) { return blocksFor(nElem); }
 
-size_t stageFusedLdsBytes(int maxSlots, int maxHalo) {
+size_t stageFusedLdsBytes(int maxSlots, int maxHalo, int layout) {
 	// the flux region also stages the block's conservative states (5 per cell) before the fluxes
+	const size_t cap = (size_t) kBlock + maxHalo;
 	const size_t flux = (size_t) 5 * (maxSlots > kBlock ? maxSlots : kBlock);
-	const size_t words = (size_t) 6 * (kBlock + maxHalo) + (size_t) 2 * kBlock + flux;
+	const size_t words = 6 * cap + (layout == 2 ? cap + kBlock : (size_t) 2 * kBlock) + flux;
 	return sizeof(double) * words;
 }
 
@@ -1068,7 +1096,7 @@ void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState 
                       int machDerived, double *rmsPartial, int parity, const StageTuning &t, hipStream_t st) {
 	if (p.nBlocks == 0) return;
 	const int rev = t.alternate ? (parity & 1) : 0;
-	const size_t lds = stageFusedLdsBytes(p.maxSlots, p.maxHalo);
+	const size_t lds = stageFusedLdsBytes(p.maxSlots, p.maxHalo, integ == 1 ? 0 : layout);
 	const int x = t.xcds;
 #define E3D_LAUNCH(S, N, K)                                                                                    \
 	do {                                                                                                       \
