@@ -9,7 +9,9 @@
 //
 // Cell state is array-of-structures, 8 doubles = 64 B per cell (rho, u, v, w, p, E, H, M): a
 // face reading a neighbour cell touches one 64 B chunk with four 16 B loads instead of six
-// scattered cache lines.
+// scattered cache lines. Between RK stages the fused kernel keeps the interior cells as the
+// conservative state instead (rho, rho u, rho v, rho w, rho E: 40 B per cell) and forms the
+// primitives on load with the update's own expressions (primOf), which gives the same bits.
 //
 // Two forms of an RK stage exist:
 //   kStageFused  the production path: one workgroup per block of kCellsPerBlock consecutive cells
@@ -643,7 +645,10 @@ __device__ __forceinline__ int logicalBlock(int w, int nBlocks, int xcds, bool r
 // is issued before any flux is computed, so a block waits for memory once: the block's own cells
 // and its halo (the outside cells its faces read) are staged in LDS as rho, u, v, w, p, H; the
 // update's inputs (W0, V, sum|S|, face slots) are loaded into registers. The flux loop then reads
-// LDS only, except for boundary faces, whose ghost cells live in global memory.
+// LDS only, except for boundary faces, whose ghost cells live in global memory. LAYOUT says where
+// the interior state is read from and written to (0: primitive records in and out, 1: primitive
+// records in and conservative states out, 2: conservative states in and out); ghost records are
+// read from `in` and written to `out` in every layout.
 template<int SCHEME, bool NT, bool MK, int INTEG, int LAYOUT>
 __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out,
                                                       const double *Win, double *Wout, DevFree q, double *W0,
