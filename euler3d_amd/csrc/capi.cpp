@@ -89,6 +89,7 @@ struct e3d_ctx {
 	DBuf<double> wst[2];
 	bool wMode = false;
 	DBuf<double> W0, fs, partial, scalars;
+	int partialCount = 0; // block partials the last RMS-forming stage wrote
 	// residual smoothing (RESIDUAL_SMOOTHING= 1): neighbour lists in device numbering, residual
 	// before/during/after the sweeps, per-block largest change, decision and NaN flags
 	int nbrStride = 0;
@@ -178,6 +179,7 @@ void runStage(e3d_ctx *c, int stage, double *rmsPartial) {
 		record(evB);
 		c->cur ^= 1;
 		c->wMode = layout != 0;
+		if (rmsPartial) c->partialCount = c->plan.nBlocks;
 	} else {
 		// EulerSolver::Run / RungeKutta body: updateBC, computeResidual, smoothResiduals, update in place
 		c->ensurePrims();
@@ -195,6 +197,7 @@ void runStage(e3d_ctx *c, int stage, double *rmsPartial) {
 		e3d::launchStage(c->dm, s, c->fr, c->fs.p, Rin, c->W0.p, alphas[stage], stage == 0, integ, rmsPartial, nullptr,
 		                 c->stream);
 		record(evB);
+		if (rmsPartial) c->partialCount = e3d::stageBlocks(c->nElem);
 	}
 	// the stored interior Mach number becomes c/|u| with the first update; TIME_INTEGRATION= 1
 	// never updates the state
@@ -233,7 +236,7 @@ void runIteration(e3d_ctx *c, double *dsum) {
 	} else {
 		runStage(c, 0, c->partial.p); // explicit Euler step, or (implicit) residual only
 	}
-	e3d::launchSum(c->partial.p, e3d::stageBlocks(c->nElem), dsum, c->stream);
+	e3d::launchSum(c->partial.p, c->partialCount, dsum, c->stream);
 }
 
 }// namespace
@@ -645,9 +648,15 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		if (const char *v = std::getenv("E3D_XCDS")) c->tune.xcds = std::max(1, std::atoi(v));
 		if (const char *v = std::getenv("E3D_SHARED_RECIP")) c->tune.sharedRecip = std::atoi(v) != 0;
 		if (const char *v = std::getenv("E3D_ALTERNATE")) c->tune.alternate = std::atoi(v) != 0;
-		e3d::HostPlan P = e3d::buildPlan(sm, order, e3d::kCellsPerBlock);
-		if (std::max(e3d::stageFusedLdsBytes(P.maxSlots, P.maxHalo, 1), e3d::stageFusedLdsBytes(P.maxSlots, P.maxHalo, 2)) >
-		    160 * 1024)
+		// blocks of 256 cells, or of 128 where a 256-cell block does not fit in a CU's LDS;
+		// E3D_BLOCK_CELLS=128|256 forces one. (Smaller blocks keep more waves resident on
+		// hexahedral meshes with large halos, but the extra halo cells and duplicated faces cost
+		// more: the 1.18M-hex wing ran 2.48 s against 2.27 s per 2,000 iterations.)
+		int cells = 0;
+		if (const char *v = std::getenv("E3D_BLOCK_CELLS")) cells = std::atoi(v) == 128 ? 128 : 256;
+		e3d::HostPlan P = e3d::buildPlan(sm, order, cells ? cells : e3d::kCellsPerBlock);
+		if (!cells && e3d::stageFusedWavesPerCU(256, P.maxSlots, P.maxHalo, p->scheme) == 0) P = e3d::buildPlan(sm, order, 128);
+		if (e3d::stageFusedWavesPerCU(P.cellsPerBlock, P.maxSlots, P.maxHalo, p->scheme) == 0)
 			return fail(E3D_ERR_UNSUPPORTED, "a block of the fused stage needs more LDS than a CU has");
 		c->new2old = P.new2old;
 
@@ -682,7 +691,7 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		c->haloCell.alloc(std::max<size_t>(1, P.haloCell.size())), c->haloCell.upload(P.haloCell.data(), P.haloCell.size(), s);
 		c->plan = {P.nBlocks, P.maxSlots, P.csrStride, c->blockSlot.p, c->se0.p, c->se1.p, c->skind.p,
 		           c->snx.p, c->sny.p, c->snz.p, c->sS.p, c->csr.p, c->spack.p, c->blockHalo.p, c->haloCell.p,
-		           P.maxHalo};
+		           P.maxHalo, P.cellsPerBlock};
 		// partitions: wall ranges, interface ghosts
 		c->nPart = sm.partStart.size() > 1 ? (int) sm.partStart.size() - 1 : 1;
 		if (!m->ranges.empty()) {
@@ -741,7 +750,7 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 			c->sync();
 		}
 		c->fs.alloc(5 * nF);
-		c->partial.alloc(std::max(1, e3d::stageBlocks(c->nElem)));
+		c->partial.alloc(std::max({1, e3d::stageBlocks(c->nElem), P.nBlocks}));
 		c->scalars.alloc(16);
 
 		c->dm.nElem = c->nElem;

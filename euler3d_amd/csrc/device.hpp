@@ -73,6 +73,7 @@ struct DevPlan {
 	const int *blockHalo;    // nBlocks + 1
 	const int *haloCell;     // outside cells read by each block, device ids
 	int maxHalo;
+	int cellsPerBlock;       // 256, or 128 where a 256-cell block needs too much LDS
 };
 
 void launchBc(const DevMesh &m, DevState s, const DevFree &q, int machDerived, hipStream_t st);
@@ -95,7 +96,9 @@ void launchSmooth(const double *R0, double *R1, double *R2, const int *nbr, cons
                   double *blockMax, int *flag, const int *partStart, int nPart, double *cellMax, hipStream_t st);
 int stageBlocks(int nElem);
 // dynamic LDS of one fused-stage workgroup for a plan's largest block, per state layout
-size_t stageFusedLdsBytes(int maxSlots, int maxHalo, int layout);
+size_t stageFusedLdsBytes(int cellsPerBlock, int maxSlots, int maxHalo, int layout);
+// waves the fused RK stage keeps resident per CU for such a plan (0: a block does not fit)
+int stageFusedWavesPerCU(int cellsPerBlock, int maxSlots, int maxHalo, int scheme);
 // launch choices of the fused stage (they change speed only, never results)
 struct StageTuning {
 	bool nontemporal = true; // non-temporal loads for arrays read once per stage

@@ -22,6 +22,8 @@
 //                per-face parity entry points (update_bc, face_flux, residual, timestep).
 #include "device.hpp"
 
+#include <algorithm>
+
 namespace e3d {
 
 namespace {
@@ -395,6 +397,18 @@ __global__ void __launch_bounds__(kBlock) kTimestep(DevMesh m, DevState s, DevFr
 }
 
 // deterministic block reduction of one double per thread (fixed tree)
+// the same fixed tree for a workgroup of N threads
+template<int N>
+__device__ __forceinline__ double blockSumN(double x, double *lds) {
+	lds[threadIdx.x] = x;
+	__syncthreads();
+	for (int off = N / 2; off > 0; off >>= 1) {
+		if ((int) threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+		__syncthreads();
+	}
+	return lds[0];
+}
+
 __device__ __forceinline__ double blockSum(double x, double *lds) {
 	lds[threadIdx.x] = x;
 	__syncthreads();
@@ -649,27 +663,27 @@ __device__ __forceinline__ int logicalBlock(int w, int nBlocks, int xcds, bool r
 // the interior state is read from and written to (0: primitive records in and out, 1: primitive
 // records in and conservative states out, 2: conservative states in and out); ghost records are
 // read from `in` and written to `out` in every layout.
-template<int SCHEME, bool NT, bool MK, int INTEG, int LAYOUT>
-__global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out,
+template<int SCHEME, bool NT, bool MK, int INTEG, int LAYOUT, int CB>
+__global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out,
                                                       const double *Win, double *Wout, DevFree q, double *W0,
                                                       double alpha, int firstStage, int machDerived,
                                                       double *rmsPartial, int xcds, int reverse) {
 	extern __shared__ double lds[];
 	const int b = logicalBlock(blockIdx.x, p.nBlocks, xcds, reverse != 0);
 	const int t = threadIdx.x;
-	const int c0 = b * kBlock;
-	const int nc = min(kBlock, m.nElem - c0);
+	const int c0 = b * CB;
+	const int nc = min(CB, m.nElem - c0);
 	const int s0 = p.blockSlot[b], ns = p.blockSlot[b + 1] - s0;
 	const int h0 = p.blockHalo[b], nh = p.blockHalo[b + 1] - h0;
-	const int cap = kBlock + p.maxHalo;
+	const int cap = CB + p.maxHalo;
 	const int ms = p.maxSlots;
 	// LDS: [6][cap] rho, u, v, w, p, H of own cells then halo cells; with LAYOUT 2 a row [cap] of
-	// one per-cell value of the flux (cellRow) and [kBlock] E of own cells, else [2][kBlock] E, M
+	// one per-cell value of the flux (cellRow) and [CB] E of own cells, else [2][CB] E, M
 	// of own cells; then [5][ms] F*S per slot (first the own W of LAYOUT 2)
 	double *ls = lds;
 	double *lcr = ls + 6 * cap;
 	double *lem = LAYOUT == 2 ? lcr + cap : ls + 6 * cap;
-	double *lf = LAYOUT == 2 ? lem + kBlock : lem + 2 * kBlock;
+	double *lf = LAYOUT == 2 ? lem + CB : lem + 2 * CB;
 	// Roe: sqrt(rho), AUSM: the sound speed sqrt(gamma p / rho), as the flux forms them
 	auto cellRow = [&](double r, double pr) -> double {
 		return (SCHEME == 0 && kRoeRow == 0) ? sqrt(r) : sqrt(q.gamma * pr / r);
@@ -687,7 +701,7 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 		const double2 *blk = reinterpret_cast<const double2 *>(cellPtr(in, c0));
 #pragma unroll
 		for (int i = 0; i < 4; i++) {
-			const int j = i * kBlock + t;
+			const int j = i * CB + t;
 			oc4[i] = j < 4 * nc ? blk[j] : make_double2(0, 0);
 		}
 	} else {
@@ -697,7 +711,7 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 		const double2 *blk = reinterpret_cast<const double2 *>(Win + (size_t) 5 * c0);
 #pragma unroll
 		for (int i = 0; i < 3; i++) {
-			const int j = i * kBlock + t;
+			const int j = i * CB + t;
 			oc4[i] = 2 * j < 5 * nc ? blk[j] : make_double2(0, 0);
 		}
 		oc4[3] = make_double2(0, 0);
@@ -725,8 +739,8 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 			for (int i = 0; i < 6; i++) code[i] = i < p.csrStride ? ldStream<NT>(cl + i) : (uint16_t) 0xFFFF;
 		}
 	}
-	for (int h = t; h < nh; h += kBlock) {
-		const int l = kBlock + h;
+	for (int h = t; h < nh; h += CB) {
+		const int l = CB + h;
 		if constexpr (LAYOUT < 2) {
 			const double2 *hc = reinterpret_cast<const double2 *>(cellPtr(in, p.haloCell[h0 + h]));
 			const double2 a0 = hc[0], a1 = hc[1], a2 = hc[2], a3 = hc[3];
@@ -754,7 +768,7 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 		// the own W range as loaded, into LDS; turned into primitives after the barrier
 #pragma unroll
 		for (int i = 0; i < 3; i++) {
-			const int j = i * kBlock + t;
+			const int j = i * CB + t;
 			if (2 * j < 5 * nc) {
 				lf[2 * j] = oc4[i].x;
 				lf[2 * j + 1] = oc4[i].y;
@@ -765,10 +779,10 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 		// second member into rows 1|3 (u, w) or the own-cell E|M rows
 		const int k = t & 3;
 		double *dA = ls + (k < 2 ? 2 * k : 2 + k) * cap;
-		double *dB = k < 2 ? ls + (2 * k + 1) * cap : lem + (k - 2) * kBlock;
+		double *dB = k < 2 ? ls + (2 * k + 1) * cap : lem + (k - 2) * CB;
 #pragma unroll
 		for (int i = 0; i < 4; i++) {
-			const int j = i * kBlock + t;
+			const int j = i * CB + t;
 			if (j < 4 * nc) {
 				dA[j >> 2] = oc4[i].x;
 				dB[j >> 2] = oc4[i].y;
@@ -806,14 +820,14 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 	auto lprim = [&](int l) -> Prim {
 		return {ls[l], ls[cap + l], ls[2 * cap + l], ls[3 * cap + l], ls[4 * cap + l], ls[5 * cap + l]};
 	};
-	for (int j = t; j < ns; j += kBlock) {
+	for (int j = t; j < ns; j += CB) {
 		const int s = s0 + j;
 		const uint32_t pk = pkN;
 		const double nx = nxN, ny = nyN, nz = nzN, S = SN;
-		if (j + kBlock < ns) {
-			pkN = ldStream<NT>(p.pack + s + kBlock);
-			nxN = ldStream<NT>(p.nx + s + kBlock), nyN = ldStream<NT>(p.ny + s + kBlock);
-			nzN = ldStream<NT>(p.nz + s + kBlock), SN = ldStream<NT>(p.S + s + kBlock);
+		if (j + CB < ns) {
+			pkN = ldStream<NT>(p.pack + s + CB);
+			nxN = ldStream<NT>(p.nx + s + CB), nyN = ldStream<NT>(p.ny + s + CB);
+			nzN = ldStream<NT>(p.nz + s + CB), SN = ldStream<NT>(p.S + s + CB);
 		}
 		const int l0 = (int) (pk & 0x7FFu), l1 = (int) ((pk >> 11) & 0x7FFu), kind = (int) (pk >> 22);
 		const Prim L = lprim(l0);
@@ -840,7 +854,7 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 			const int c = p.e1[s];    // its ghost
 			Ghost G;
 			if (kind == 3) {
-				double Mi = (LAYOUT == 2 || machDerived) ? machOf(L.rho, L.u, L.v, L.w, L.p, q.gamma) : lem[kBlock + l0];
+				double Mi = (LAYOUT == 2 || machDerived) ? machOf(L.rho, L.u, L.v, L.w, L.p, q.gamma) : lem[CB + l0];
 				G = farfieldGhost(in, c, Mi, nx, ny, nz, q, L, lem[l0]);
 			} else {
 				G = mirrorGhost(L.rho, L.u, L.v, L.w, L.p, nx, ny, nz, q.gamma);
@@ -938,7 +952,7 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 			oc[2] = make_double2(np, nE);
 			// the whole 64 B record is written (the stored Mach number carried over), so no line
 			// is left partly written
-			oc[3] = make_double2(nH, lem[kBlock + t]);
+			oc[3] = make_double2(nH, lem[CB + t]);
 		} else {
 			// the conservative state only: the next stage (or the read-back) forms the primitives
 			// from it with the same expressions
@@ -953,7 +967,7 @@ __global__ void __launch_bounds__(kBlock) kStageFused(DevMesh m, DevPlan p, DevS
 	}
 	if (rmsPartial) {
 		__syncthreads();
-		double tt = blockSum(r2, lds);
+		double tt = blockSumN<CB>(r2, lds);
 		if (t == 0) rmsPartial[b] = tt;
 	}
 }
@@ -1040,12 +1054,28 @@ inline int blocksFor(int n) { return (n + kBlock - 1) / kBlock; }
 
 int stageBlocks(int nElem) { return blocksFor(nElem); }
 
-size_t stageFusedLdsBytes(int maxSlots, int maxHalo, int layout) {
+size_t stageFusedLdsBytes(int cellsPerBlock, int maxSlots, int maxHalo, int layout) {
 	// the flux region also stages the block's conservative states (5 per cell) before the fluxes
-	const size_t cap = (size_t) kBlock + maxHalo;
-	const size_t flux = (size_t) 5 * (maxSlots > kBlock ? maxSlots : kBlock);
-	const size_t words = 6 * cap + (layout == 2 ? cap + kBlock : (size_t) 2 * kBlock) + flux;
+	const size_t cb = (size_t) cellsPerBlock, cap = cb + maxHalo;
+	const size_t flux = 5 * std::max((size_t) maxSlots, cb);
+	const size_t words = 6 * cap + (layout == 2 ? cap + cb : 2 * cb) + flux;
 	return sizeof(double) * words;
+}
+
+int stageFusedWavesPerCU(int cellsPerBlock, int maxSlots, int maxHalo, int scheme) {
+	const size_t lds = std::max(stageFusedLdsBytes(cellsPerBlock, maxSlots, maxHalo, 1),
+	                            stageFusedLdsBytes(cellsPerBlock, maxSlots, maxHalo, 2));
+	if (lds > 160 * 1024) return 0;
+	int n = 0;
+	hipError_t e;
+	if (cellsPerBlock == 128)
+		e = scheme == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kStageFused<1, true, true, 0, 2, 128>, 128, lds)
+		                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kStageFused<0, true, true, 0, 2, 128>, 128, lds);
+	else
+		e = scheme == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kStageFused<1, true, true, 0, 2, 256>, 256, lds)
+		                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kStageFused<0, true, true, 0, 2, 256>, 256, lds);
+	if (e != hipSuccess || n <= 0) n = 1;
+	return n * cellsPerBlock / 64;
 }
 
 void launchWToPrim(const DevMesh &m, const double *W, DevState s, const DevFree &q, hipStream_t st) {
@@ -1101,19 +1131,24 @@ void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState 
                       int machDerived, double *rmsPartial, int parity, const StageTuning &t, hipStream_t st) {
 	if (p.nBlocks == 0) return;
 	const int rev = t.alternate ? (parity & 1) : 0;
-	const size_t lds = stageFusedLdsBytes(p.maxSlots, p.maxHalo, integ == 1 ? 0 : layout);
+	const size_t lds = stageFusedLdsBytes(p.cellsPerBlock, p.maxSlots, p.maxHalo, integ == 1 ? 0 : layout);
 	const int x = t.xcds;
-#define E3D_LAUNCH(S, N, K)                                                                                    \
+#define E3D_LAUNCH_B(S, N, K, B)                                                                               \
 	do {                                                                                                       \
 		if (integ == 1)                                                                                        \
-			hipLaunchKernelGGL((kStageFused<S, N, K, 1, 0>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, \
+			hipLaunchKernelGGL((kStageFused<S, N, K, 1, 0, B>), dim3(p.nBlocks), dim3(B), lds, st, m, p, in, out, \
 			                   Win, Wout, q, W0, alpha, 0, machDerived, rmsPartial, x, rev);                      \
 		else if (layout == 2)                                                                                  \
-			hipLaunchKernelGGL((kStageFused<S, N, K, 0, 2>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, \
+			hipLaunchKernelGGL((kStageFused<S, N, K, 0, 2, B>), dim3(p.nBlocks), dim3(B), lds, st, m, p, in, out, \
 			                   Win, Wout, q, W0, alpha, firstStage, machDerived, rmsPartial, x, rev);             \
 		else                                                                                                   \
-			hipLaunchKernelGGL((kStageFused<S, N, K, 0, 1>), dim3(p.nBlocks), dim3(kBlock), lds, st, m, p, in, out, \
+			hipLaunchKernelGGL((kStageFused<S, N, K, 0, 1, B>), dim3(p.nBlocks), dim3(B), lds, st, m, p, in, out, \
 			                   Win, Wout, q, W0, alpha, firstStage, machDerived, rmsPartial, x, rev);             \
+	} while (0)
+#define E3D_LAUNCH(S, N, K)                                  \
+	do {                                                     \
+		if (p.cellsPerBlock == 128) E3D_LAUNCH_B(S, N, K, 128); \
+		else E3D_LAUNCH_B(S, N, K, 256);                        \
 	} while (0)
 #define E3D_LAUNCH_K(S, N)               \
 	if (t.sharedRecip) E3D_LAUNCH(S, N, true); \
@@ -1127,6 +1162,7 @@ void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState 
 	}
 #undef E3D_LAUNCH_K
 #undef E3D_LAUNCH
+#undef E3D_LAUNCH_B
 }
 
 void launchSum(const double *in, int n, double *out, hipStream_t st) {

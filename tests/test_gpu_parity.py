@@ -65,9 +65,14 @@ def test_gpu_bc_flux_residual_dt_iterate(case, tmp_path):
             assert abs(sums[0] - ref) <= 1e-12 * abs(ref), (sums[0], ref)
 
 
+@pytest.mark.parametrize("cells", ["auto", "128", "256"])
 @pytest.mark.parametrize("case", SMALL_CASES)
-def test_gpu_run_matches_reference(case, tmp_path):
-    say(f"[gpu] run case={case}")
+def test_gpu_run_matches_reference(case, cells, tmp_path, monkeypatch):
+    """A whole run against the reference, with the fused stage's blocks of the size the library
+    picks and of both sizes forced."""
+    say(f"[gpu] run case={case} cells={cells}")
+    if cells != "auto":
+        monkeypatch.setenv("E3D_BLOCK_CELLS", cells)
     g, pd, m, s = make_solver(case, tmp_path)
     with s:
         r = s.run()

@@ -71,7 +71,7 @@ def test_gpu_partitioned_bc_flux_residual_dt_iterate(case, tmp_path):
             assert abs(sums[0] - ref) <= 1e-12 * abs(ref), (sums[0], ref)
 
 
-@pytest.mark.parametrize("path", ["fused", "unfused"])
+@pytest.mark.parametrize("path", ["fused", "fused256", "unfused"])
 @pytest.mark.parametrize("case", RUN_CASES)
 def test_gpu_partitioned_run_matches_reference(case, path, tmp_path, monkeypatch):
     """A whole multi-rank run on one device: final state of every partition bit for bit, the
@@ -79,6 +79,8 @@ def test_gpu_partitioned_run_matches_reference(case, path, tmp_path, monkeypatch
     say(f"[gpu] partitioned run case={case} path={path}")
     if path == "unfused":
         monkeypatch.setenv("E3D_STAGE_PATH", "unfused")
+    if path == "fused256":
+        monkeypatch.setenv("E3D_BLOCK_CELLS", "256")
     g = load_golden(case)
     pd = integrator_params(g)
     m = composite(case, tmp_path)
