@@ -114,13 +114,15 @@ __device__ __forceinline__ void roeFlux(const Prim &L, const Prim &R, double nx,
 // Quotient x / d given y = RN(1 / d), correctly rounded (Markstein, IBM J. Res. Dev. 34 (1990)
 // 111-119): q = RN(x y) is within one ulp of x / d, the remainder x - q d is exact under an FMA,
 // and RN(q + (x - q d) y) is RN(x / d). Several quotients by one denominator then cost one IEEE
-// division plus a multiply and two FMAs each. A zero quotient is returned as q so that its sign
-// matches IEEE division. (Finite, normal operands; the solver's values are.)
+// division plus a multiply and two FMAs each. The result takes q's sign (one bitfield insert), so
+// that a zero quotient keeps the sign IEEE division gives it. (Finite, normal operands; the
+// solver's values are.)
 __device__ __forceinline__ double divRecip(double x, double d, double y) {
 	const double q = x * y;
 	const double e = fma(-q, d, x);
 	const double r = fma(e, y, q);
-	return q == 0.0 ? q : r;
+	// r has the sign of q whenever q is not zero; for a zero q (x = +-0) the sign must be q's
+	return copysign(r, q);
 }
 
 // x / d either as the IEEE division or through the shared reciprocal y of d
