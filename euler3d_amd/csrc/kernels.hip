@@ -29,9 +29,6 @@ namespace e3d {
 namespace {
 
 constexpr int kBlock = 256;
-// the per-cell value the fused stage keeps in LDS for the Roe flux with conservative states in:
-// 0 sqrt(rho) (two square roots less per face), 1 the sound speed (a division and a root less)
-constexpr int kRoeRow = 0;
 static_assert(kBlock == kCellsPerBlock, "one thread per cell in the fused stage");
 
 struct Prim {
@@ -679,16 +676,19 @@ __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState
 	const int h0 = p.blockHalo[b], nh = p.blockHalo[b + 1] - h0;
 	const int cap = CB + p.maxHalo;
 	const int ms = p.maxSlots;
-	// LDS: [6][cap] rho, u, v, w, p, H of own cells then halo cells; with LAYOUT 2 a row [cap] of
-	// one per-cell value of the flux (cellRow) and [CB] E of own cells, else [2][CB] E, M
-	// of own cells; then [5][ms] F*S per slot (first the own W of LAYOUT 2)
+	// LDS: [6][cap] rho, u, v, w, p, H of own cells then halo cells; with LAYOUT 2 two more rows
+	// [cap], sqrt(rho) and the sound speed sqrt(gamma p / rho) as the fluxes and the time step form
+	// them, else [2][CB] E, M of own cells; then [5][ms] F*S per slot (first the own W of LAYOUT 2)
 	double *ls = lds;
-	double *lcr = ls + 6 * cap;
-	double *lem = LAYOUT == 2 ? lcr + cap : ls + 6 * cap;
-	double *lf = LAYOUT == 2 ? lem + CB : lem + 2 * CB;
-	// Roe: sqrt(rho), AUSM: the sound speed sqrt(gamma p / rho), as the flux forms them
-	auto cellRow = [&](double r, double pr) -> double {
-		return (SCHEME == 0 && kRoeRow == 0) ? sqrt(r) : sqrt(q.gamma * pr / r);
+	double *lsq = ls + 6 * cap, *lcs = lsq + cap;
+	double *lem = ls + 6 * cap;
+	double *lf = LAYOUT == 2 ? lcs + cap : lem + 2 * CB;
+	double Eown = 0.0; // LAYOUT 2: E of this thread's cell
+	// LAYOUT 2: E of own cell l from its conservative state, as primOf forms it
+	auto ownE = [&](int l) -> double {
+		const double *w5 = Win + (size_t) 5 * (c0 + l);
+		const double r = w5[0];
+		return quot<MK>(w5[4], r, MK ? 1.0 / r : 0.0);
 	};
 	const size_t n = (size_t) m.nElem;
 
@@ -763,7 +763,8 @@ __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState
 			ls[3 * cap + l] = w;
 			ls[4 * cap + l] = pr;
 			ls[5 * cap + l] = H;
-			lcr[l] = cellRow(r, pr);
+			lsq[l] = sqrt(r);
+			lcs[l] = sqrt(q.gamma * pr / r);
 		}
 	}
 	if constexpr (LAYOUT == 2) {
@@ -812,8 +813,9 @@ __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState
 			ls[3 * cap + t] = w;
 			ls[4 * cap + t] = pr;
 			ls[5 * cap + t] = H;
-			lcr[t] = cellRow(r, pr);
-			lem[t] = E;
+			lsq[t] = sqrt(r);
+			lcs[t] = sqrt(q.gamma * pr / r);
+			Eown = E;
 		}
 		__syncthreads();
 	}
@@ -837,10 +839,8 @@ __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState
 		if (SCHEME == 0 && kind == 0) {
 			const Prim R = lprim(l1);
 			double sL, sR, cL;
-			if constexpr (LAYOUT == 2 && kRoeRow == 0) {
-				sL = lcr[l0], sR = lcr[l1], cL = sqrt(q.gamma * L.p / L.rho);
-			} else if constexpr (LAYOUT == 2) {
-				sL = sqrt(L.rho), sR = sqrt(R.rho), cL = lcr[l0];
+			if constexpr (LAYOUT == 2) {
+				sL = lsq[l0], sR = lsq[l1], cL = lcs[l0];
 			} else {
 				sL = sqrt(L.rho), sR = sqrt(R.rho), cL = sqrt(q.gamma * L.p / L.rho);
 			}
@@ -849,7 +849,7 @@ __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState
 		}
 		double F[5];
 		if (kind == 0) {
-			if constexpr (SCHEME == 1 && LAYOUT == 2) ausmFluxA(L, lprim(l1), lcr[l0], lcr[l1], nx, ny, nz, q.gamma, F);
+			if constexpr (SCHEME == 1 && LAYOUT == 2) ausmFluxA(L, lprim(l1), lcs[l0], lcs[l1], nx, ny, nz, q.gamma, F);
 			else schemeFlux<SCHEME>(L, lprim(l1), nx, ny, nz, q.gamma, F);
 		} else {
 			const int a = c0 + l0;    // the interior cell of a boundary face is an own cell
@@ -857,7 +857,7 @@ __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState
 			Ghost G;
 			if (kind == 3) {
 				double Mi = (LAYOUT == 2 || machDerived) ? machOf(L.rho, L.u, L.v, L.w, L.p, q.gamma) : lem[CB + l0];
-				G = farfieldGhost(in, c, Mi, nx, ny, nz, q, L, lem[l0]);
+				G = farfieldGhost(in, c, Mi, nx, ny, nz, q, L, LAYOUT == 2 ? ownE(l0) : lem[l0]);
 			} else {
 				G = mirrorGhost(L.rho, L.u, L.v, L.w, L.p, nx, ny, nz, q.gamma);
 			}
@@ -911,13 +911,13 @@ __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState
 			w0[1] = u * rho;
 			w0[2] = v * rho;
 			w0[3] = w * rho;
-			w0[4] = lem[t] * rho;
+			w0[4] = (LAYOUT == 2 ? Eown : lem[t]) * rho;
 #pragma unroll
 			for (int k = 0; k < 5; k++) W0[k * n + e] = w0[k];
 		}
 		// TimeStepMethod2 (TimeIntegration.cpp:8-40) and FlowField::updateWRungeKutta
 		// (FlowField.cpp:140-158), as updateCell
-		double c = sqrt(q.gamma * pp / rho);
+		double c = LAYOUT == 2 ? lcs[t] : sqrt(q.gamma * pp / rho);
 		double lx = 0.5 * (fabs(u) + c) * sx;
 		double ly = 0.5 * (fabs(v) + c) * sy;
 		double lz = 0.5 * (fabs(w) + c) * sz;
@@ -1060,7 +1060,7 @@ size_t stageFusedLdsBytes(int cellsPerBlock, int maxSlots, int maxHalo, int layo
 	// the flux region also stages the block's conservative states (5 per cell) before the fluxes
 	const size_t cb = (size_t) cellsPerBlock, cap = cb + maxHalo;
 	const size_t flux = 5 * std::max((size_t) maxSlots, cb);
-	const size_t words = 6 * cap + (layout == 2 ? cap + cb : 2 * cb) + flux;
+	const size_t words = 6 * cap + (layout == 2 ? 2 * cap : 2 * cb) + flux;
 	return sizeof(double) * words;
 }
 
