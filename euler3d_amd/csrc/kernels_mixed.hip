@@ -237,13 +237,14 @@ __device__ __forceinline__ void storeRec(float *rec, int i, const D &d, float M)
 	p[1] = make_float4(d.p, d.H, M, 0.0f);
 }
 
-__device__ __forceinline__ int logicalBlock(int w, int n, int xcds) {
-	if (xcds <= 1) return w;
+// (case, block) pair of walk index w: each XCD takes a contiguous range of pairs; with reverse it
+// walks its range from the end, so that a stage starts on the cells the previous one wrote last
+__device__ __forceinline__ int logicalBlock(int w, int n, int xcds, bool reverse) {
+	if (xcds <= 1) return reverse ? n - 1 - w : w;
 	const int x = w % xcds, k = w / xcds;
 	const int q = n / xcds, r = n % xcds;
-	// dispatch slots beyond n (when n is not a multiple of xcds) keep their index order
-	const int idx = x * q + min(x, r) + k;
-	return idx;
+	const int len = q + (x < r ? 1 : 0);
+	return x * q + min(x, r) + (reverse ? len - 1 - k : k);
 }
 
 __device__ __forceinline__ double blockSumD(double x, double *lds) {
@@ -272,7 +273,7 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 		if (gb >= nb) break;
 		// the tables are read-only during the launch: read them through the constant address
 		// space so that the per-block descriptor lands in scalar registers
-		const int lb = logicalBlock(gb, nb, xcds);
+		const int lb = logicalBlock(gb, nb, xcds, (stage & 1) != 0);
 		const int cslot = ((const kConst int *) L.blocks)[2 * lb], b = ((const kConst int *) L.blocks)[2 * lb + 1];
 		const kConst MixCase &C = ((const kConst MixCase *) L.cases)[cslot];
 		const kConst MixStatus &stt = ((const kConst MixStatus *) L.status)[cslot];
