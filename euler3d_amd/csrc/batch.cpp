@@ -751,6 +751,7 @@ int e3d_batch_get_state(e3d_batch *b, int id, double *const fields[8]) {
 		const size_t nE = (size_t) c.nElem, nG = (size_t) c.nGhost;
 		double *tmp = nullptr;
 		hipCheck(hipMallocAsync((void **) &tmp, 8 * nE * sizeof(double), b->stream), "hipMallocAsync");
+		if (h->iter > 0) e3d::launchMixMirrorGhosts(c.desc, h->parity, b->stream);
 		e3d::launchMixPrimitives(c.desc, tmp, b->stream);
 		hipCheck(hipGetLastError(), "kernel launch");
 		std::vector<double> prim(8 * nE);

@@ -340,7 +340,6 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 				const D Lc = ld(l0);
 				const int g = C.ghostOf[s];
 				const GhostD G = mirrorD(Lc, nx, ny, nz, f);
-				if (stage == 4) storeRec(out, g, G.d, G.M);
 				fluxDelta(SCHEME, Lc, G.d, nx, ny, nz, f, F);
 			}
 #pragma unroll
@@ -387,9 +386,9 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 					const int g = C.ghostOf[s];
 					GhostD G = kind == 3 ? farfieldD(Lc, rb.z, recPtr(in, g), nx, ny, nz, f) : mirrorD(Lc, nx, ny, nz, f);
 					// a farfield ghost's state is read by the next stage's classifier; wall and
-					// symmetry ghosts are rebuilt from their cell every stage and only kept for
-					// read-back, so they are stored by the iteration's last stage only
-					if (kind == 3 || stage == 4) storeRec(out, g, G.d, G.M);
+					// symmetry ghosts are rebuilt from their cell every stage and nothing reads
+					// them back while iterating, so kMixMirrorGhosts forms them for read-back only
+					if (kind == 3) storeRec(out, g, G.d, G.M);
 					const float Vinf = f.u * nx + f.v * ny + f.w * nz;
 					const float minf = f.r * Vinf;
 					float F[5];
@@ -512,6 +511,27 @@ __global__ void __launch_bounds__(kEndThreads) kMixIterEnd(const MixCase *cases,
 	}
 }
 
+// Wall and symmetry ghosts as the last stage built them, for read-back (the stage kernel does not
+// store them): the mirror of the cell state that stage started from, rec[par ^ 1], written into
+// the current buffer rec[par]. One workgroup per block; a boundary face's cell is an own cell.
+__global__ void __launch_bounds__(kB) kMixMirrorGhosts(MixCase C, int par) {
+	const int b = blockIdx.x;
+	const int s0 = C.blockSlot[b], ns = C.blockSlot[b + 1] - s0;
+	const Free32 f = {C.rInf, C.uInf, C.vInf, C.wInf, C.pInf, C.hInf, C.mInf, C.gamma, C.gamma / (C.gamma - 1.0f)};
+	const float *in = C.rec[par ^ 1];
+	float *out = C.rec[par];
+	for (int j = threadIdx.x; j < ns; j += kB) {
+		const int s = s0 + j;
+		const uint32_t pk = C.pack[s];
+		const int kind = (int) (pk >> 22);
+		if (kind != 1 && kind != 2) continue;
+		const float *r = recPtr(in, b * kB + (int) (pk & 0x7FFu));
+		const D Lc = {r[R_DRHO], r[R_DU], r[R_DV], r[R_DW], r[R_DP], r[R_DH]};
+		const GhostD G = mirrorD(Lc, C.nx[s], C.ny[s], C.nz[s], f);
+		storeRec(out, C.ghostOf[s], G.d, G.M);
+	}
+}
+
 // FP64 primitives of the interior cells from W
 __global__ void __launch_bounds__(256) kMixPrimitives(MixCase C, double *out) {
 	const int e = blockIdx.x * 256 + threadIdx.x;
@@ -607,6 +627,11 @@ int mixResidentPerCU(int ms, int cap, int schemes, bool nontemporal) {
 void launchMixIterEnd(const MixLaunch &L, hipStream_t st) {
 	if (L.nSlots <= 0) return;
 	hipLaunchKernelGGL(kMixIterEnd, dim3(L.nSlots), dim3(kEndThreads), 0, st, L.cases, L.status);
+}
+
+void launchMixMirrorGhosts(const MixCase &hc, int parity, hipStream_t st) {
+	if (hc.nBlocks <= 0 || hc.nGhost <= 0) return;
+	hipLaunchKernelGGL(kMixMirrorGhosts, dim3(hc.nBlocks), dim3(kB), 0, st, hc, parity);
 }
 
 void launchMixPrimitives(const MixCase &hc, double *out, hipStream_t st) {

@@ -101,6 +101,8 @@ void launchMixStage(const MixLaunch &L, int stage, const MixTuning &t, hipStream
 // after the 5 stages: per-case RMS, history, convergence, buffer parity
 void launchMixIterEnd(const MixLaunch &L, hipStream_t st);
 // FP64 primitives (rho, u, v, w, p, E, H, M) of the interior cells from W: out[8][nElem]
+// wall and symmetry ghost records of the last completed iteration into rec[parity]
+void launchMixMirrorGhosts(const MixCase &hc, int parity, hipStream_t st);
 void launchMixPrimitives(const MixCase &hc, double *out, hipStream_t st);
 // wall-face force/moment of one case from its FP64 state (same reduction as the FP64 path)
 void launchMixAero(int nWall, const int *wallCell, const double *Nx, const double *Ny, const double *Nz,
