@@ -187,6 +187,32 @@ def test_gpu_reads_between_iterations_change_nothing(scheme, tmp_path):
     assert bits_equal(cf3, cf_ref) and bits_equal(cm3, cm_ref)
 
 
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_gpu_launch_switches_do_not_change_results(scheme, monkeypatch):
+    """Each launch-shape and arithmetic-sequence switch of the fused stage (library IEEE divisions
+    instead of the shared-reciprocal and short sequences, cached loads, plain dispatch order, one
+    walk direction, Morton cell order, 128-cell blocks) gives the state bits of the default."""
+    say(f"[gpu] launch switches scheme={scheme}")
+    m = Mesh.bump(nx=24, ny=8, nz=8, jitter=0.1, seed=0, permute=True).preprocess(1.0)
+    pd = golden_params(load_golden("bump_small"))
+
+    def run():
+        with EulerSolver(m, native_params(pd, scheme=scheme)) as s:
+            sums = s.iterate(12)
+            return sums, s.get_state()
+
+    base_sums, base = run()
+    for var, val in (("E3D_SHARED_RECIP", "0"), ("E3D_NONTEMPORAL", "0"), ("E3D_XCDS", "1"), ("E3D_ALTERNATE", "0"),
+                     ("E3D_CELL_ORDER", "morton"), ("E3D_BLOCK_CELLS", "128")):
+        with monkeypatch.context() as mp:
+            mp.setenv(var, val)
+            sums, st = run()
+        for k in FIELDS:
+            assert bits_equal(st[k], base[k]), (var, k)
+        # the RMS sums block partials, which follow the cell order and block size
+        np.testing.assert_allclose(sums, base_sums, rtol=1e-12, err_msg=var)
+
+
 def test_gpu_freestream_preserved_and_conservative(tmp_path):
     """Size-independent properties on a mid-size permuted tet mesh: a uniform state is a steady
     state of the interior scheme (closed cells), and the sum of all cell residuals equals the sum of
