@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import argparse
 import json
+from concurrent.futures import ThreadPoolExecutor
 import os
 import sys
 import time
@@ -159,7 +160,9 @@ def bench_batch(a) -> dict:
     with Batch(max_cases=len(cases), wg_per_cu=tune["best"][0], xcds=tune["best"][1],
                nontemporal=tune["best"][2]) as b:
         t0 = time.perf_counter()
-        ids = [b.add(m, p, drop=a.drop, max_iter=a.max_iter) for m, p in built]
+        # admission (plan and upload) runs on worker threads, as in the server
+        with ThreadPoolExecutor(a.admit_workers) as pool:
+            ids = list(pool.map(lambda mp: b.add(mp[0], mp[1], drop=a.drop, max_iter=a.max_iter), built))
         t_admit = time.perf_counter() - t0
         while True:
             b.run(100)

@@ -175,7 +175,7 @@ std::vector<int> bisectOrder(const SolverMesh &sm, int cellsPerBlock) {
 		select(best);
 		for (int i = lo; i < hi; i++) ids[i] = kv[i - lo].id;
 		std::vector<KV>().swap(kv);
-		if (depth < 4) {
+		if (depth < 4 && hi - lo > (1 << 16)) {
 			std::thread left([&, lo, cut, depth] { run(lo, cut, depth + 1); });
 			run(cut, hi, depth + 1);
 			left.join();
