@@ -1096,6 +1096,24 @@ int e3d_checkpoint_read(e3d_ctx *c, const char *path) {
 	});
 }
 
+int e3d_math_selfcheck(int device, long long n, unsigned long long seed, unsigned long long mismatches[3]) {
+	if (!mismatches || n < 0) return fail(E3D_ERR_ARG, "bad argument");
+	return guarded([&] {
+		hipCheck(hipSetDevice(device), "hipSetDevice");
+		unsigned long long *bad = nullptr;
+		hipCheck(hipMalloc((void **) &bad, 3 * sizeof(unsigned long long)), "hipMalloc");
+		hipError_t e = hipMemset(bad, 0, 3 * sizeof(unsigned long long));
+		if (e == hipSuccess) {
+			e3d::launchMathCheck(seed, n, bad, nullptr);
+			e = hipGetLastError();
+		}
+		if (e == hipSuccess) e = hipMemcpy(mismatches, bad, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+		(void) hipFree(bad);
+		hipCheck(e, "math self-check");
+		return E3D_OK;
+	});
+}
+
 int e3d_last_elapsed_ms(e3d_ctx *c, double *ms) {
 	if (!c || !ms) return fail(E3D_ERR_ARG, "null argument");
 	*ms = c->lastMs;

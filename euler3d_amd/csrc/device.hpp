@@ -126,6 +126,9 @@ struct StageTuning {
 void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const double *Win, double *Wout,
                       int layout, const DevFree &q, int scheme, int integ, double *W0, double alpha, int firstStage,
                       int machDerived, double *rmsPartial, int parity, const StageTuning &t, hipStream_t st);
+// bitwise comparison of the short square root, reciprocal and quotient with the library's on n
+// pseudo-random inputs; bad[0..2] (device, zeroed by the caller) count the mismatches
+void launchMathCheck(unsigned long long seed, long long n, unsigned long long *bad, hipStream_t st);
 // interior primitive records of s from the conservative states W (the stage's expressions), with
 // M = c / |u|
 void launchWToPrim(const DevMesh &m, const double *W, DevState s, const DevFree &q, hipStream_t st);

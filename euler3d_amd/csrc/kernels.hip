@@ -1026,6 +1026,37 @@ __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState
 	}
 }
 
+// self-check of sqrtN, recipN and divN against the library square root and division (bitwise)
+__device__ __forceinline__ uint64_t mixBits(uint64_t z) {
+	z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+	z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+	return z ^ (z >> 31);
+}
+__device__ __forceinline__ double randDouble(uint64_t z, int span, bool sign) {
+	const uint64_t ex = 1023 - span + (z >> 53) % (2 * span + 1);
+	uint64_t b = (ex << 52) | (z & 0xFFFFFFFFFFFFFull);
+	if (sign && ((z >> 40) & 1)) b |= 0x8000000000000000ull;
+	return __longlong_as_double((long long) b);
+}
+__global__ void __launch_bounds__(kBlock) kMathCheck(uint64_t seed, uint64_t base, long long n,
+                                                      unsigned long long *bad) {
+	const uint64_t i = base + (uint64_t) blockIdx.x * kBlock + threadIdx.x;
+	if ((long long) i >= n) return;
+	const uint64_t z = mixBits(seed ^ (3 * i)), z1 = mixBits(seed ^ (3 * i + 1)), z2 = mixBits(seed ^ (3 * i + 2));
+	const double xs = randDouble(z, 700, false);
+	const double d = randDouble(z1, 400, true);
+	double x = randDouble(z2, 400, true);
+	if ((i & 15) == 0) x = 0.0;
+	if ((i & 15) == 1) x = -0.0;
+	unsigned int m = 0;
+	if (__double_as_longlong(sqrtN(xs)) != __double_as_longlong(sqrt(xs))) m |= 1;
+	if (__double_as_longlong(recipN(d)) != __double_as_longlong(1.0 / d)) m |= 2;
+	if (__double_as_longlong(divN(x, d)) != __double_as_longlong(x / d)) m |= 4;
+	if (m & 1) atomicAdd(bad, 1ull);
+	if (m & 2) atomicAdd(bad + 1, 1ull);
+	if (m & 4) atomicAdd(bad + 2, 1ull);
+}
+
 // interior primitives (64 B records) from the conservative states the fused RK stage keeps
 __global__ void __launch_bounds__(kBlock) kWToPrim(DevMesh m, const double *W, DevState s, DevFree q) {
 	const int e = blockIdx.x * kBlock + threadIdx.x;
@@ -1130,6 +1161,15 @@ int stageFusedWavesPerCU(int cellsPerBlock, int maxSlots, int maxHalo, int schem
 		                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kStageFused<0, true, true, 0, 2, 256>, 256, lds);
 	if (e != hipSuccess || n <= 0) n = 1;
 	return n * cellsPerBlock / 64;
+}
+
+void launchMathCheck(unsigned long long seed, long long n, unsigned long long *bad, hipStream_t st) {
+	const long long perLaunch = (long long) kBlock * 65536;
+	for (long long base = 0; base < n; base += perLaunch) {
+		const long long cnt = std::min(perLaunch, n - base);
+		hipLaunchKernelGGL(kMathCheck, dim3((unsigned) ((cnt + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, (uint64_t) seed,
+		                   (uint64_t) base, n, bad);
+	}
 }
 
 void launchWToPrim(const DevMesh &m, const double *W, DevState s, const DevFree &q, hipStream_t st) {

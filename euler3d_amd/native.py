@@ -65,6 +65,7 @@ IP = C.POINTER(C.c_int)
 _SIGS = {
     "e3d_last_error": (C.c_char_p, []),
     "e3d_version": (C.c_char_p, []),
+    "e3d_math_selfcheck": (C.c_int, [C.c_int, C.c_longlong, C.c_ulonglong, C.POINTER(C.c_ulonglong)]),
     "e3d_params_default": (C.c_int, [C.POINTER(Params)]),
     "e3d_config_read": (C.c_int, [C.c_char_p, C.POINTER(Params), C.c_char_p, C.c_int, C.c_char_p, C.c_int]),
     "e3d_config_partition": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p, C.c_int, IP]),

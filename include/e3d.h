@@ -202,6 +202,14 @@ int e3d_last_elapsed_ms(e3d_ctx *c, double *ms);
  * name, then clears the record. No counterpart in the reference (measurement hook). */
 int e3d_set_kernel_timing(e3d_ctx *c, int enable);
 int e3d_kernel_timing(e3d_ctx *c, double *avg_ms, int *launches, char *name, int name_len);
+
+/* Self-check of the short FP64 sequences the fused stage uses in place of the library's square
+ * root and division (no operand scaling): on device `device`, n pseudo-random inputs from `seed`
+ * (square roots of positive values 2^-700 .. 2^700; reciprocals and quotients of operands
+ * 2^-400 .. 2^400 of either sign, an eighth of the numerators +-0) are compared bit for bit with
+ * the library results. mismatches[0..2] receive the counts for the square root, the reciprocal and
+ * the quotient. No counterpart in the reference (a check of this library's arithmetic). */
+int e3d_math_selfcheck(int device, long long n, unsigned long long seed, unsigned long long mismatches[3]);
 void e3d_destroy(e3d_ctx *c);
 
 /* ---- mixed-precision multi-case engine (batched and served solves) ------------------------

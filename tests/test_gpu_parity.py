@@ -213,6 +213,19 @@ def test_gpu_launch_switches_do_not_change_results(scheme, monkeypatch):
         np.testing.assert_allclose(sums, base_sums, rtol=1e-12, err_msg=var)
 
 
+def test_gpu_short_sqrt_and_division_sequences_are_ieee():
+    """The fused stage's square root, reciprocal and quotient without operand scaling give the
+    library's (IEEE) bits on 2^26 pseudo-random inputs each, zeros of either sign included."""
+    import ctypes as C
+
+    from euler3d_amd import native as N
+
+    say("[gpu] short sqrt/div sequences")
+    bad = (C.c_ulonglong * 3)()
+    N.check(N.lib().e3d_math_selfcheck(0, 1 << 26, 12345, bad))
+    assert list(bad) == [0, 0, 0], list(bad)
+
+
 def test_gpu_freestream_preserved_and_conservative(tmp_path):
     """Size-independent properties on a mid-size permuted tet mesh: a uniform state is a steady
     state of the interior scheme (closed cells), and the sum of all cell residuals equals the sum of
