@@ -16,7 +16,8 @@ from pathlib import Path
 
 from .solver import Batch
 
-CANDIDATES = tuple((w, x, nt) for w in (2, 3, 4, 5, 6, 8) for x in (8, 1) for nt in (True, False))
+# wg_per_cu 0: one workgroup per (case, block) pair; else a persistent grid of that many per CU
+CANDIDATES = tuple((w, x, nt) for w in (0, 2, 3, 4, 5, 6, 8) for x in (8, 1) for nt in (True, False))
 
 
 def cache_path() -> Path:

@@ -207,11 +207,18 @@ void activate(e3d_batch *b) {
 		L.cap = e3d::kMixBlock + k.maxHalo;
 		L.ms = (k.maxSlots + 1) & ~1;
 		L.schemes = 1 << k.scheme;
-		// never more persistent workgroups than are resident at once
-		const int perCU = std::min(std::max(1, b->tune.wgPerCU), e3d::mixResidentPerCU(L.ms, L.cap, L.schemes,
-		                                                                                b->tune.nontemporal));
-		const int maxGrid = std::max(xc, b->nCU * perCU / xc * xc);
-		L.grid = (int) std::min<size_t>((size_t) maxGrid, (k.blocks + xc - 1) / xc * xc);
+		// wg_per_cu 0: one workgroup per (case, block) pair, left to the hardware scheduler (on the
+		// 6.3M-tet bump 251.6 us per stage against 262.1 us for the best persistent grid); the grid
+		// is rounded up to a multiple of 2048 (surplus workgroups exit at once) so that admissions
+		// and retirements rarely change it and force a new graph. Else a persistent grid, never
+		// more workgroups than are resident at once.
+		const size_t gran = (size_t) std::max(xc, 2048);
+		L.grid = (int) ((k.blocks + gran - 1) / gran * gran);
+		if (b->tune.wgPerCU > 0) {
+			const int perCU = std::min(b->tune.wgPerCU, e3d::mixResidentPerCU(L.ms, L.cap, L.schemes, b->tune.nontemporal));
+			const int maxGrid = std::max(xc, b->nCU * perCU / xc * xc);
+			L.grid = std::min(L.grid, maxGrid);
+		}
 		Ls.push_back(L);
 	}
 	if (n) hipCheck(hipMemcpyAsync(b->dBlocks, b->hBlocks, n * sizeof(int2), hipMemcpyHostToDevice, b->stream), "H2D blocks");
@@ -297,7 +304,7 @@ int e3d_batch_opts_default(e3d_batch_opts *o) {
 	std::memset(o, 0, sizeof(*o));
 	o->hbm_budget = 0.0;
 	o->max_cases = 256;
-	o->wg_per_cu = 8; // clamped to what fits on a CU at once
+	o->wg_per_cu = 0; // one workgroup per (case, block) pair
 	o->xcds = 8;
 	o->nontemporal = 1;
 	o->use_graph = 1;
@@ -310,7 +317,7 @@ int e3d_batch_create(int device, const e3d_batch_opts *opts, e3d_batch **out) {
 	e3d_batch_opts_default(&o);
 	if (opts) o = *opts;
 	if (o.max_cases <= 0 || o.max_cases > 65536) return fail(E3D_ERR_ARG, "max_cases must be in 1..65536");
-	if (o.wg_per_cu <= 0 || o.wg_per_cu > 32) return fail(E3D_ERR_ARG, "wg_per_cu must be in 1..32");
+	if (o.wg_per_cu < 0 || o.wg_per_cu > 32) return fail(E3D_ERR_ARG, "wg_per_cu must be in 0..32");
 	if (o.xcds <= 0 || o.xcds > 64) return fail(E3D_ERR_ARG, "xcds must be in 1..64");
 	return guarded([&] {
 		auto b = std::make_unique<e3d_batch>();
@@ -798,7 +805,7 @@ int e3d_batch_remove(e3d_batch *b, int id) {
 
 int e3d_batch_set_tuning(e3d_batch *b, int wgPerCU, int xcds, int nontemporal) {
 	if (!b) return fail(E3D_ERR_ARG, "null batch");
-	if (wgPerCU <= 0 || wgPerCU > 32 || xcds <= 0 || xcds > 64) return fail(E3D_ERR_ARG, "invalid launch shape");
+	if (wgPerCU < 0 || wgPerCU > 32 || xcds <= 0 || xcds > 64) return fail(E3D_ERR_ARG, "invalid launch shape");
 	b->tune.wgPerCU = wgPerCU;
 	b->tune.xcds = xcds;
 	b->tune.nontemporal = nontemporal != 0;

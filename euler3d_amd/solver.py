@@ -292,7 +292,7 @@ class Batch:
     one persistent launch per RK stage (no counterpart in the reference, which solves one case per
     process). A case's result does not depend on the other resident cases."""
 
-    def __init__(self, device: int = 0, hbm_budget: float = 0.0, max_cases: int = 256, wg_per_cu: int = 8,
+    def __init__(self, device: int = 0, hbm_budget: float = 0.0, max_cases: int = 256, wg_per_cu: int = 0,
                  xcds: int = 8, nontemporal: bool = True, use_graph: bool = True):
         o = N.BatchOpts()
         N.check(N.lib().e3d_batch_opts_default(C.byref(o)))

@@ -228,7 +228,8 @@ typedef struct e3d_batch e3d_batch;
 typedef struct e3d_batch_opts {
 	double hbm_budget;   /* bytes the resident cases may hold; <= 0: 90 % of the free device memory */
 	int max_cases;       /* case slots */
-	int wg_per_cu;       /* persistent workgroups per compute unit (launch shape; results unchanged) */
+	int wg_per_cu;       /* 0: one workgroup per (case, block) pair; else persistent workgroups per
+	                        compute unit (launch shape; results unchanged) */
 	int xcds;            /* XCD count of the L2-aware block order, 1 = dispatch order */
 	int nontemporal;     /* non-temporal loads for read-once arrays */
 	int use_graph;       /* replay iterations from a captured HIP graph */
@@ -269,7 +270,7 @@ int e3d_batch_result(e3d_batch *b, int case_id, double *hist, int cap, int *n_hi
 int e3d_batch_get_state(e3d_batch *b, int case_id, double *const fields[8]);
 /* Retires a case and releases its device memory (stream-ordered; no device-wide wait). */
 int e3d_batch_remove(e3d_batch *b, int case_id);
-/* Changes the launch shape (persistent workgroups per CU, XCD order, non-temporal loads) of the
+/* Changes the launch shape (0 or persistent workgroups per CU, XCD order, non-temporal loads) of the
  * next iterations; results are bitwise the same for every shape. Used by the autotuner. */
 int e3d_batch_set_tuning(e3d_batch *b, int wg_per_cu, int xcds, int nontemporal);
 /* resident cases, bytes they hold, budget */

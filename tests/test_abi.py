@@ -59,7 +59,7 @@ def test_batch_case_bytes_and_argument_checks():
     lib = native.lib()
     o = native.BatchOpts()
     assert lib.e3d_batch_opts_default(ctypes.byref(o)) == 0
-    assert o.max_cases > 0 and o.wg_per_cu > 0 and o.use_graph == 1
+    assert o.max_cases > 0 and o.wg_per_cu == 0 and o.use_graph == 1  # 0: one workgroup per (case, block)
     o.max_cases = 0
     h = ctypes.c_void_p()
     assert lib.e3d_batch_create(0, ctypes.byref(o), ctypes.byref(h)) == 1
