@@ -1,6 +1,7 @@
 """Launch-shape autotuner of the mixed-precision multi-case engine.
 
-The persistent stage kernel's launch shape (workgroups per CU, XCD-aware block order,
+The stage kernel's launch shape (one workgroup per block or a persistent grid of so many per CU,
+XCD-aware block order,
 non-temporal loads for read-once arrays) changes speed only: every shape gives bitwise the same
 results. The tuner admits the workload's cases into a throwaway engine, times a few iterations per
 candidate shape and keeps the fastest. Results are cached per device and workload class in a JSON

@@ -1,5 +1,5 @@
 // Mixed-precision multi-case engine behind the e3d_batch_* entry points: device memory of the
-// resident cases, the case and block tables the persistent stage kernel walks, admission against
+// resident cases, the case and block tables the stage kernel walks, admission against
 // the HBM budget, stream-ordered retirement, and the captured HIP graph of one iteration.
 #include "abi_util.hpp"
 #include "mixed.hpp"
@@ -98,7 +98,7 @@ struct e3d_batch {
 	// guards the host-side tables: admissions may run on other threads while iterations run
 	std::mutex mu;
 	// launch description and the captured iteration
-	// one persistent launch per class of cases: cases are grouped by flux scheme and by how many
+	// one launch per class of cases: cases are grouped by flux scheme and by how many
 	// of their workgroups fit on a CU (their LDS need), so that a case with large blocks does not
 	// lower the occupancy of every other case
 	std::vector<e3d::MixLaunch> Ls;

@@ -1,6 +1,6 @@
 // HIP kernels of the mixed-precision multi-case solver for gfx950 (wave64).
 //
-// One persistent launch per RK stage walks the (case, block) table. A block of kMixBlock
+// One launch per RK stage walks the (case, block) table. A block of kMixBlock
 // consecutive cells of one case is processed as in the FP64 fused stage: own and halo cells are
 // staged in LDS, every face touching the block is evaluated once (FP32, perturbation form), then
 // one thread per cell gathers its faces in ascending reference face id with a Kahan-compensated

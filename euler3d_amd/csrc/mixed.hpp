@@ -15,7 +15,7 @@
 //
 // Cases. Every resident case owns its arrays (plan, metrics, W, two FP32 record buffers,
 // per-block partial sums, residual history). A device table of MixCase descriptors and a table
-// of (case, block) pairs let one persistent launch walk the blocks of all cases; a case's
+// of (case, block) pairs let one launch walk the blocks of all cases; a case's
 // arithmetic does not depend on which other cases are resident or where its blocks sit in the
 // table, so its result is the same alone and in any mix.
 #pragma once
@@ -74,7 +74,7 @@ struct MixStatus {
 
 // launch choices of the mixed stage (they change speed only, never results)
 struct MixTuning {
-	int wgPerCU = 4;        // persistent workgroups per compute unit
+	int wgPerCU = 0;        // 0: one workgroup per (case, block) pair; else persistent workgroups per CU
 	int xcds = 8;           // XCD count for the L2-aware block order
 	bool nontemporal = true;
 };
@@ -85,7 +85,7 @@ struct MixLaunch {
 	const int2 *blocks;     // (case slot, block of the case)
 	const int *nBlocks;     // number of entries of `blocks` in use (device scalar)
 	int nSlots;             // case slots
-	int grid;               // persistent workgroups (a multiple of the XCD count)
+	int grid;               // workgroups (a multiple of the XCD count)
 	int cap;                // LDS cells per block: kMixBlock + the largest halo of any resident case
 	int ms;                 // LDS face slots per block: the largest slot count, even
 	int schemes;            // bit s set: some resident case uses flux scheme s (one launch each)

@@ -289,7 +289,7 @@ class EulerSolver:
 
 class Batch:
     """Mixed-precision multi-case engine: many cases resident on one device, advanced together by
-    one persistent launch per RK stage (no counterpart in the reference, which solves one case per
+    one launch per RK stage (no counterpart in the reference, which solves one case per
     process). A case's result does not depend on the other resident cases."""
 
     def __init__(self, device: int = 0, hbm_budget: float = 0.0, max_cases: int = 256, wg_per_cu: int = 0,

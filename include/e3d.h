@@ -214,8 +214,8 @@ void e3d_destroy(e3d_ctx *c);
 
 /* ---- mixed-precision multi-case engine (batched and served solves) ------------------------
  * No counterpart in the reference, which runs one case per process (src/Solver.cpp:19-73). Many
- * cases of any sizes stay resident on one device and advance together: one persistent kernel
- * launch per RK stage walks the blocks of every resident case, and one iteration (5 stages plus
+ * cases of any sizes stay resident on one device and advance together: one kernel launch per RK
+ * stage covers the blocks of every resident case (one workgroup per block by default), and one iteration (5 stages plus
  * the per-case RMS and convergence test) is replayed from a captured HIP graph. Face fluxes,
  * ghosts and local time steps are FP32 (perturbation form, on FP32 copies of the state and the
  * face geometry); the conservative state, the RK update, the Kahan-compensated residual
