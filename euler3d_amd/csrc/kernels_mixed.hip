@@ -8,6 +8,12 @@
 //
 // FP32 quotients and square roots use the hardware approximations (v_rcp_f32, v_sqrt_f32, 1 ulp):
 // their error is of the order of the FP32 rounding of the inputs.
+//
+// The library is compiled with -ffp-contract=off for the bit-exact FP64 path. The FP32 interior
+// fluxes (physDelta, roeDelta, ausmDelta) turn it back on locally: a multiply-add becomes one
+// FMA (about 1 % of the stage time). Every term of those fluxes carries a perturbation factor, so
+// a freestream state still gives an exact zero. The ghost states, the time step and the FP64
+// update keep separately rounded products.
 #include "mixed.hpp"
 
 namespace e3d {
@@ -53,6 +59,7 @@ __device__ __forceinline__ float deltaH(float dr, float du, float dv, float dw, 
 // dm = rho_inf dV + d_rho V.
 __device__ __forceinline__ void physDelta(const D &d, float nx, float ny, float nz, float Vinf, float minf,
                                           const Free32 &f, float out[5]) {
+#pragma clang fp contract(fast)
 	const float dV = d.u * nx + d.v * ny + d.w * nz;
 	const float V = Vinf + dV;
 	const float dm = f.r * dV + d.r * V;
@@ -69,6 +76,7 @@ __device__ __forceinline__ void physDelta(const D &d, float nx, float ny, float 
 // jumps of the primitives.
 __device__ __forceinline__ void roeDelta(const D &L, const D &R, float nx, float ny, float nz, const Free32 &f,
                                          float out[5]) {
+#pragma clang fp contract(fast)
 	const float Vinf = f.u * nx + f.v * ny + f.w * nz;
 	const float minf = f.r * Vinf;
 	float FL[5], FR[5];
@@ -138,6 +146,7 @@ __device__ __forceinline__ float dpMinus(float M, float N, float dM) {
 
 __device__ __forceinline__ void ausmDelta(const D &L, const D &R, float nx, float ny, float nz, const Free32 &f,
                                           float out[5]) {
+#pragma clang fp contract(fast)
 	const float aInf = fsqrt(f.g * f.p * frcp(f.r));
 	const float Vinf = f.u * nx + f.v * ny + f.w * nz;
 	const float Minf = Vinf * frcp(aInf);
