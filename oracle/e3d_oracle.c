@@ -167,15 +167,41 @@ void or_roe(const or_mesh *m, const or_params *q, const or_state *s, int face, d
 	for (int i = 0; i < 5; i++) out[i] = avg[i] - 0.5 * (f1[i] + f234[i] + f5[i]);
 }
 
-/* AUSM, Liou & Steffen (1993): Mach-number and pressure splitting of degree 2, convective
- * quantities upwinded by the sign of the interface Mach number. PARITY UNPINNED. */
+/* AUSM, Liou & Steffen, J. Comput. Phys. 107 (1993) 23-39: Mach-number splitting of degree 2
+ * and pressure splitting of degree 3, convective quantities Phi = rho a (1, u, v, w, H) upwinded by
+ * the sign of the interface Mach number m = M+(M_L) + M-(M_R), M = V / a with each side's own
+ * sound speed. The reference only declares it (ConvectiveFluxesSchemes.h:27-30); what pins it is
+ * listed in e3d_oracle.h.
+ *
+ * OR_AUSM_MUTANT (compile-time, tests only) plants one plausible mistake so that the tests can show
+ * that their checks catch it: 1 swapped pressure splittings, 2 the left sound speed for both sides,
+ * 3 the |m| dissipation term dropped, 4 the pressure polynomial (2 - M) written (2 + M). */
+#ifndef OR_AUSM_MUTANT
+#define OR_AUSM_MUTANT 0
+#endif
 static double mach_plus(double M) { return fabs(M) <= 1.0 ? 0.25 * (M + 1.0) * (M + 1.0) : 0.5 * (M + fabs(M)); }
 static double mach_minus(double M) { return fabs(M) <= 1.0 ? -0.25 * (M - 1.0) * (M - 1.0) : 0.5 * (M - fabs(M)); }
 static double pres_plus(double M) {
+#if OR_AUSM_MUTANT == 4
+	return fabs(M) <= 1.0 ? 0.25 * (M + 1.0) * (M + 1.0) * (2.0 + M) : (M > 0.0 ? 1.0 : 0.0);
+#else
 	return fabs(M) <= 1.0 ? 0.25 * (M + 1.0) * (M + 1.0) * (2.0 - M) : (M > 0.0 ? 1.0 : 0.0);
+#endif
 }
 static double pres_minus(double M) {
+#if OR_AUSM_MUTANT == 4
+	return fabs(M) <= 1.0 ? 0.25 * (M - 1.0) * (M - 1.0) * (2.0 - M) : (M < 0.0 ? 1.0 : 0.0);
+#else
 	return fabs(M) <= 1.0 ? 0.25 * (M - 1.0) * (M - 1.0) * (2.0 + M) : (M < 0.0 ? 1.0 : 0.0);
+#endif
+}
+
+/* The four splittings at M: out = {M+, M-, P+, P-} (for the identity checks of the tests). */
+void or_ausm_split(double M, double out[4]) {
+	out[0] = mach_plus(M);
+	out[1] = mach_minus(M);
+	out[2] = pres_plus(M);
+	out[3] = pres_minus(M);
 }
 
 void or_ausm(const or_mesh *m, const or_params *q, const or_state *s, int face, double out[5]) {
@@ -185,16 +211,28 @@ void or_ausm(const or_mesh *m, const or_params *q, const or_state *s, int face, 
 	double rL = s->rho[L], rR = s->rho[R], pL = s->p[L], pR = s->p[R];
 	double uL = s->u[L], vL = s->v[L], wL = s->w[L], hL = s->H[L];
 	double uR = s->u[R], vR = s->v[R], wR = s->w[R], hR = s->H[R];
+#if OR_AUSM_MUTANT == 2
+	double aL = sqrt(g * pL / rL), aR = aL;
+#else
 	double aL = sqrt(g * pL / rL), aR = sqrt(g * pR / rR);
+#endif
 	double VL = uL * n[0] + vL * n[1] + wL * n[2];
 	double VR = uR * n[0] + vR * n[1] + wR * n[2];
 	double ML = VL / aL, MR = VR / aR;
 	double mh = mach_plus(ML) + mach_minus(MR);
+#if OR_AUSM_MUTANT == 1
+	double ph = pres_minus(ML) * pL + pres_plus(MR) * pR;
+#else
 	double ph = pres_plus(ML) * pL + pres_minus(MR) * pR;
+#endif
 	double phiL[5] = {rL * aL, rL * aL * uL, rL * aL * vL, rL * aL * wL, rL * aL * hL};
 	double phiR[5] = {rR * aR, rR * aR * uR, rR * aR * vR, rR * aR * wR, rR * aR * hR};
 	double pn[5] = {0.0, ph * n[0], ph * n[1], ph * n[2], 0.0};
+#if OR_AUSM_MUTANT == 3
+	for (int i = 0; i < 5; i++) out[i] = mh * 0.5 * (phiL[i] + phiR[i]) + pn[i];
+#else
 	for (int i = 0; i < 5; i++) out[i] = mh * 0.5 * (phiL[i] + phiR[i]) - 0.5 * fabs(mh) * (phiR[i] - phiL[i]) + pn[i];
+#endif
 }
 
 /* Solver::Fc, ConvectiveFluxesSchemes.cpp:167-187 */

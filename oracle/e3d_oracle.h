@@ -26,9 +26,14 @@
  *                      stops (MPIHandler.cpp:16-313, FlowField.cpp:129-169)
  *                                                  golden fixtures of mpiexec -np N runs (r<r>_*)
  *   or_ausm            no reference implementation exists (ConvectiveFluxesSchemes.h:27-30 only
- *                      declares it): PARITY UNPINNED. Restates Liou & Steffen, J. Comput. Phys.
- *                      107 (1993) 23-39, and is checked by invariants only (consistency
- *                      F(U,U,n) = physical flux, supersonic upwinding, conservation).
+ *                      declares it), so no fixture can pin it. It restates Liou & Steffen,
+ *                      J. Comput. Phys. 107 (1993) 23-39, and is pinned by what the mathematics
+ *                      fixes (tests/test_ausm_invariants.py): the splittings are the unique
+ *                      polynomials with M+ + M- = M, P+ + P- = 1 and C1 joins at |M| = 1;
+ *                      consistency F(U,U,n) = physical flux; exact upwinding for M_L, M_R > 1 and
+ *                      < -1; mirror antisymmetry F(L,R,n) = -F(R,L,-n) (bitwise); rotation
+ *                      invariance; conservation on a closed mesh. Each planted mistake
+ *                      (OR_AUSM_MUTANT 1-4) fails at least one of them.
  */
 #ifndef E3D_ORACLE_H
 #define E3D_ORACLE_H
@@ -87,6 +92,8 @@ void or_init_state(const or_mesh *m, const or_params *prm, or_state *s);
 void or_update_bc(const or_mesh *m, const or_params *prm, or_state *s);
 void or_roe(const or_mesh *m, const or_params *prm, const or_state *s, int face, double out[5]);
 void or_ausm(const or_mesh *m, const or_params *prm, const or_state *s, int face, double out[5]);
+/* {M+, M-, P+, P-} of or_ausm's splittings at Mach number M */
+void or_ausm_split(double M, double out[4]);
 void or_fc(const or_mesh *m, const or_state *s, int ghost, int face, double V, double out[5]);
 /* per-face flux as the residual dispatches it; F unscaled and FS = F*S (5 per face), kind per face
  * (0 interior, 1 wall, 2 symmetry, 3 farfield) */
