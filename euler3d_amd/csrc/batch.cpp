@@ -452,6 +452,9 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 		initCaseHost(P, sm, *p, *co, lay);
 		{
 			std::lock_guard<std::mutex> lk(b->mu);
+			if ((double) lay.total > b->budget)
+				return fail(E3D_ERR_ARG, "the case needs " + std::to_string(lay.total) +
+				                             " B, more than the whole HBM budget; it can never be admitted");
 			if (b->used + (double) lay.total > b->budget)
 				return fail(E3D_ERR_BUSY, "HBM budget exhausted (" + std::to_string(lay.total) + " B requested)");
 			b->used += (double) lay.total;
@@ -671,6 +674,9 @@ int e3d_batch_status(e3d_batch *b, int id, int *done, int *iters, double *rms0, 
 		if (iters) *iters = h->iter;
 		if (rms0) *rms0 = h->rms0;
 		if (rmsLast) *rmsLast = h->rmsLast;
+		if (h->done == E3D_CASE_FAILED)
+			return fail(E3D_ERR_NUMERIC, "case " + std::to_string(id) + ": non-finite residual at iteration " +
+			                                 std::to_string(h->iter));
 		return E3D_OK;
 	});
 }
@@ -740,6 +746,9 @@ int e3d_batch_result(e3d_batch *b, int id, double *hist, int cap, int *nHist, do
 			if (cfOut) cfOut[k] = cf[k];
 			if (cmOut) cmOut[k] = cm[k];
 		}
+		if (h->done == E3D_CASE_FAILED)
+			return fail(E3D_ERR_NUMERIC, "case " + std::to_string(id) + ": non-finite residual at iteration " +
+			                                 std::to_string(h->iter));
 		return E3D_OK;
 	});
 }

@@ -108,6 +108,8 @@ struct e3d_ctx {
 	std::vector<int> wall0, nWallPart;
 	std::vector<int> mpiGhost, mpiCell;
 	DBuf<double> aeroOut;
+	DBuf<double> wallCp;  // cp of every wall face at the last force update (AeroCoefficients::_cp)
+	bool wallCpSet = false;
 
 	e3d::DevState state(int which) { return {st[which].p}; }
 	e3d::DevState state() { return state(cur); }
@@ -245,6 +247,15 @@ extern "C" {
 
 const char *e3d_last_error(void) { return e3d_abi::lastError().c_str(); }
 const char *e3d_version(void) { return "e3d-mi355x 0.1 (gfx950, FP64 plain path)"; }
+
+int e3d_device_count(int *n) {
+	if (!n) return fail(E3D_ERR_ARG, "null argument");
+	int k = 0;
+	// no device (or no driver) is a count of zero, not an error
+	if (hipGetDeviceCount(&k) != hipSuccess) k = 0;
+	*n = k;
+	return E3D_OK;
+}
 
 int e3d_params_default(e3d_params *out) {
 	if (!out) return fail(E3D_ERR_ARG, "null params");
@@ -704,6 +715,7 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		// wall list in the reference's order for the force integration
 		c->nWall = (int) P.wallCell.size();
 		if (c->nWall) {
+			c->wallCp.alloc(c->nWall);
 			c->wallCell.alloc(c->nWall), c->wallCell.upload(P.wallCell.data(), c->nWall, s);
 			for (int k = 0; k < 3; k++) {
 				c->wN[k].alloc(c->nWall), c->wN[k].upload(P.wallN[k].data(), c->nWall, s);
@@ -933,13 +945,14 @@ int e3d_aero(e3d_ctx *c, double cf[3], double cm[3]) {
 			e3d::launchAero(c->nWallPart[r], c->wallCell.p + w0, c->wN[0].p + w0, c->wN[1].p + w0, c->wN[2].p + w0,
 			                c->wC[0].p + w0, c->wC[1].p + w0, c->wC[2].p + w0, c->state(),
 			                c->wMode ? c->wst[c->cur].p : nullptr, c->fr.gamma, c->cpFactor, c->fr.p_inf,
-			                c->prm.cref, c->prm.sref, c->aeroOut.p + 6 * (size_t) r, c->stream);
+			                c->prm.cref, c->prm.sref, c->aeroOut.p + 6 * (size_t) r, c->wallCp.p + w0, c->stream);
 			c->check();
 			any = true;
 		}
 		if (any) {
 			hipCheck(hipMemcpyAsync(h.data(), c->aeroOut.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream), "D2H");
 			c->sync();
+			c->wallCpSet = true;
 		}
 		for (int r = 0; r < c->nPart; r++)
 			if (!c->nWallPart[r]) std::fill(h.begin() + 6 * r, h.begin() + 6 * r + 6, 0.0); // no wall face: zero sums
@@ -954,6 +967,24 @@ int e3d_aero(e3d_ctx *c, double cf[3], double cm[3]) {
 				cm[k] = r ? cm[k] + mk : mk;
 			}
 		}
+		return E3D_OK;
+	});
+}
+
+int e3d_wall_cp(e3d_ctx *c, double *cp, long long capacity, long long *n) {
+	if (!c) return fail(E3D_ERR_ARG, "null context");
+	return guarded([&] {
+		if (n) *n = c->nWall;
+		if (!cp || !c->nWall) return E3D_OK;
+		if (capacity < c->nWall) return fail(E3D_ERR_ARG, "wall cp: capacity below the wall face count");
+		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		if (!c->wallCpSet) {
+			// AeroCoefficients' cp vector before its first Update: zeros (AeroCoefficients.cpp:27)
+			std::fill(cp, cp + c->nWall, 0.0);
+			return E3D_OK;
+		}
+		hipCheck(hipMemcpyAsync(cp, c->wallCp.p, sizeof(double) * c->nWall, hipMemcpyDeviceToHost, c->stream), "D2H");
+		c->sync();
 		return E3D_OK;
 	});
 }

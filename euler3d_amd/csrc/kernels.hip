@@ -1100,7 +1100,7 @@ __global__ void __launch_bounds__(kBlock) kMach(DevMesh m, DevState s, DevFree q
 __global__ void __launch_bounds__(kBlock) kAero(int nWall, const int *wallCell, const double *Nx, const double *Ny,
                                                 const double *Nz, const double *Cx, const double *Cy, const double *Cz,
                                                 DevState s, const double *W, double gamma, double cpFactor, double pInf,
-                                                double cref, double sref, double *out6) {
+                                                double cref, double sref, double *out6, double *cpOut) {
 	__shared__ double lds[kBlock];
 	double acc[6] = {0, 0, 0, 0, 0, 0};
 	int per = (nWall + kBlock - 1) / kBlock;
@@ -1116,6 +1116,7 @@ __global__ void __launch_bounds__(kBlock) kAero(int nWall, const int *wallCell, 
 			pw = cellPtr(s, wallCell[i])[S_P];
 		}
 		double cp = cpFactor * (pw - pInf);
+		if (cpOut) cpOut[i] = cp;
 		double cx = Nx[i] * cp * cref / sref;
 		double cy = Ny[i] * cp * cref / sref;
 		double cz = Nz[i] * cp * cref / sref;
@@ -1270,9 +1271,9 @@ void launchMach(const DevMesh &m, DevState s, const DevFree &q, hipStream_t st) 
 
 void launchAero(int nWall, const int *wallCell, const double *Nx, const double *Ny, const double *Nz, const double *Cx,
                 const double *Cy, const double *Cz, DevState s, const double *W, double gamma, double cpFactor,
-                double pInf, double cref, double sref, double *out6, hipStream_t st) {
+                double pInf, double cref, double sref, double *out6, double *cpOut, hipStream_t st) {
 	hipLaunchKernelGGL(kAero, dim3(1), dim3(kBlock), 0, st, nWall, wallCell, Nx, Ny, Nz, Cx, Cy, Cz, s, W, gamma,
-	                   cpFactor, pInf, cref, sref, out6);
+	                   cpFactor, pInf, cref, sref, out6, cpOut);
 }
 
 }// namespace e3d

@@ -515,8 +515,12 @@ __global__ void __launch_bounds__(kEndThreads) kMixIterEnd(const MixCase *cases,
 		st.rmsLast = rms;
 		st.iter = it + 1;
 		st.parity ^= 1;
-		const bool conv = (C.drop > 0.0 && rms < C.drop * r0) || rms < C.minResidual || !(rms == rms);
-		if (conv || it + 1 >= C.maxIter) st.done = 1;
+		// a NaN or infinite residual retires the case as failed; it never counts as converged
+		const bool bad = !(rms - rms == 0.0);
+		const bool conv = (C.drop > 0.0 && rms < C.drop * r0) || rms < C.minResidual;
+		if (bad) st.done = 3;
+		else if (conv) st.done = 1;
+		else if (it + 1 >= C.maxIter) st.done = 2;
 	}
 }
 

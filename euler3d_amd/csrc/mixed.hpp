@@ -67,7 +67,8 @@ struct MixCase {
 struct MixStatus {
 	int parity; // FP32 record buffer holding the state at the start of an iteration
 	int iter;   // iterations done
-	int done;   // 1: converged or out of iterations (its blocks are skipped)
+	int done;   // nonzero: finished, its blocks are skipped. 1 converged, 2 iteration cap, 3 non-finite
+	            // residual (E3D_CASE_CONVERGED, _CAPPED, _FAILED)
 	int active; // 0: the slot holds no case
 	double rms0, rmsLast;
 };

@@ -128,12 +128,10 @@ int main(int argc, char **argv) {
 	e3d_mesh_get_f64(mesh, "faceCenters", fc.data(), (long long) fc.size(), &n);
 	e3d_mesh_get_f64(mesh, "faceNormals", fn.data(), (long long) fn.size(), &n);
 
-	// cp of the last coefficient update (AeroCoefficients::Update); zero when none happened
-	const double pi = 3.141592653589793238;
-	double uinf = prm.mach * std::sqrt(prm.gamma) * std::cos(prm.aoa_deg * (pi / 180));
-	double vinf = prm.mach * std::sqrt(prm.gamma) * std::sin(prm.aoa_deg * (pi / 180));
-	double cpFactor = 1 / (0.5 * 1.0 * (uinf * uinf + vinf * vinf + 0.0 * 0.0));
-	bool sampled = iters >= prm.sampling;
+	// cp of the last coefficient update (AeroCoefficients::Update at the last sampled iteration, or
+	// zeros when none happened), as PrintCp writes it (EulerSolver.cpp:282-300)
+	std::vector<double> wallCp(info.n_wall);
+	if (e3d_wall_cp(ctx, wallCp.data(), info.n_wall, &n) != E3D_OK) die("wall cp read-back");
 	double wsum[3] = {0, 0, 0}; // Wall_Face_Vector_Sum: per partition, then summed in rank order
 	for (int r = 0; r < nPart; r++) {
 		const e3d_partition_info &P = parts[r];
@@ -141,7 +139,7 @@ int main(int argc, char **argv) {
 			std::ofstream f(out / ("cp_part_" + std::to_string(r) + ".dat"));
 			f << std::setw(20) << "Cp" << std::setw(20) << "x" << std::setw(20) << "y" << std::setw(20) << "z" << std::endl;
 			for (int i = P.wall0; i < P.wall0 + P.n_wall; i++) {
-				double cp = sampled ? cpFactor * (fld[4][wallCell[i]] - 1.0) : 0.0;
+				double cp = wallCp[i];
 				const double *c = &fc[3 * (size_t) wallFace[i]];
 				f << std::scientific << std::setprecision(10) << std::setw(20) << cp << std::setw(20) << c[0] << std::setw(20)
 				  << c[1] << std::setw(20) << c[2] << std::endl;

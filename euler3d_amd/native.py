@@ -15,7 +15,11 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "libe3d_hip.so"
 
 E3D_OK = 0
 ERRORS = {1: "argument", 2: "io", 3: "mesh", 4: "device", 5: "unsupported", 6: "busy", 7: "numeric"}
+E3D_ERR_ARG = 1
 E3D_ERR_BUSY = 6
+E3D_ERR_NUMERIC = 7
+# done codes of a batched case (E3D_CASE_* in include/e3d.h)
+CASE_OUTCOMES = {0: "running", 1: "converged", 2: "capped", 3: "failed"}
 
 
 class E3DError(RuntimeError):
@@ -65,6 +69,7 @@ IP = C.POINTER(C.c_int)
 _SIGS = {
     "e3d_last_error": (C.c_char_p, []),
     "e3d_version": (C.c_char_p, []),
+    "e3d_device_count": (C.c_int, [IP]),
     "e3d_math_selfcheck": (C.c_int, [C.c_int, C.c_longlong, C.c_ulonglong, C.POINTER(C.c_ulonglong)]),
     "e3d_params_default": (C.c_int, [C.POINTER(Params)]),
     "e3d_config_read": (C.c_int, [C.c_char_p, C.POINTER(Params), C.c_char_p, C.c_int, C.c_char_p, C.c_int]),
@@ -95,6 +100,7 @@ _SIGS = {
     "e3d_timestep": (C.c_int, [P, DP]),
     "e3d_iterate": (C.c_int, [P, C.c_int, DP]),
     "e3d_aero": (C.c_int, [P, DP, DP]),
+    "e3d_wall_cp": (C.c_int, [P, DP, C.c_longlong, C.POINTER(C.c_longlong)]),
     "e3d_run": (C.c_int, [P, DP, C.c_int, IP, IP, DP, DP, DP]),
     "e3d_last_elapsed_ms": (C.c_int, [P, DP]),
     "e3d_checkpoint_write": (C.c_int, [P, C.c_char_p]),
@@ -171,12 +177,9 @@ def default_params(**kw) -> Params:
 
 
 def has_device() -> bool:
-    """True when a HIP device is visible (does not initialise HIP in this process)."""
+    """True when the library sees a HIP device."""
     if os.environ.get("HIP_VISIBLE_DEVICES", None) == "":
         return False
-    try:
-        import torch  # device discovery only
-
-        return torch.cuda.device_count() > 0
-    except Exception:
-        return False
+    n = C.c_int(0)
+    check(lib().e3d_device_count(C.byref(n)))
+    return n.value > 0

@@ -232,6 +232,14 @@ class EulerSolver:
         N.check(N.lib().e3d_aero(self._h, N.dptr(cf), N.dptr(cm)))
         return cf, cm
 
+    def wall_cp(self) -> np.ndarray:
+        """cp of every wall face as the last force update left it (zeros before any)."""
+        n = C.c_longlong(0)
+        N.check(N.lib().e3d_wall_cp(self._h, None, 0, C.byref(n)))
+        cp = np.empty(n.value)
+        N.check(N.lib().e3d_wall_cp(self._h, N.dptr(cp), n.value, C.byref(n)))
+        return cp
+
     def run(self, max_rows: int | None = None) -> dict:
         rows_n = max_rows if max_rows is not None else self.params.max_iter + 1
         rows = np.zeros(8 * max(rows_n, 1))
@@ -339,10 +347,14 @@ class Batch:
         N.check(N.lib().e3d_batch_wait(self._h))
 
     def status(self, cid: int) -> dict:
+        """done: finished; outcome: running / converged / capped / failed (non-finite residual)."""
         done, it = C.c_int(), C.c_int()
         r0, r1 = C.c_double(), C.c_double()
-        N.check(N.lib().e3d_batch_status(self._h, int(cid), C.byref(done), C.byref(it), C.byref(r0), C.byref(r1)))
-        return {"done": bool(done.value), "iterations": it.value, "rms0": r0.value, "rms": r1.value}
+        rc = N.lib().e3d_batch_status(self._h, int(cid), C.byref(done), C.byref(it), C.byref(r0), C.byref(r1))
+        if rc != N.E3D_ERR_NUMERIC:
+            N.check(rc)
+        return {"done": bool(done.value), "outcome": N.CASE_OUTCOMES[done.value], "iterations": it.value,
+                "rms0": r0.value, "rms": r1.value}
 
     def poll(self) -> list[dict]:
         """Wait for the queued iterations; progress of every resident case."""
@@ -354,8 +366,8 @@ class Batch:
             N.check(N.lib().e3d_batch_poll(self._h, N.iptr(ids), N.iptr(done), N.iptr(it), N.dptr(rms), cap,
                                            C.byref(n)))
             if n.value <= cap:
-                return [{"id": int(ids[k]), "done": bool(done[k]), "iterations": int(it[k]), "rms": float(rms[k])}
-                        for k in range(n.value)]
+                return [{"id": int(ids[k]), "done": bool(done[k]), "outcome": N.CASE_OUTCOMES[int(done[k])],
+                         "iterations": int(it[k]), "rms": float(rms[k])} for k in range(n.value)]
             cap = n.value
 
     def result(self, cid: int) -> dict:
@@ -363,10 +375,12 @@ class Batch:
         hist = np.empty(max(cap, 1))
         nh = C.c_int()
         coeffs, cf, cm = np.empty(3), np.empty(3), np.empty(3)
-        N.check(N.lib().e3d_batch_result(self._h, int(cid), N.dptr(hist), cap, C.byref(nh), N.dptr(coeffs),
-                                         N.dptr(cf), N.dptr(cm)))
+        rc = N.lib().e3d_batch_result(self._h, int(cid), N.dptr(hist), cap, C.byref(nh), N.dptr(coeffs),
+                                      N.dptr(cf), N.dptr(cm))
+        if rc != N.E3D_ERR_NUMERIC:
+            N.check(rc)
         return {"rms": hist[: nh.value].copy(), "CL": coeffs[0], "CD": coeffs[1], "CM": coeffs[2],
-                "force_coeff": cf, "moment_coeff": cm}
+                "force_coeff": cf, "moment_coeff": cm, "failed": rc == N.E3D_ERR_NUMERIC}
 
     def get_state(self, cid: int) -> dict:
         n_elem, n_ghost, _ = self._n[cid]

@@ -49,7 +49,8 @@ extern "C" {
 #define E3D_ERR_DEVICE 4
 #define E3D_ERR_UNSUPPORTED 5
 #define E3D_ERR_BUSY 6 /* batch: no free case slot or the HBM budget is exhausted; retry after a removal */
-#define E3D_ERR_NUMERIC 7 /* a NaN residual in the residual smoothing (the reference exits the process) */
+#define E3D_ERR_NUMERIC 7 /* a NaN residual in the residual smoothing (the reference exits the process), or a
+                           * batched case whose residual became NaN or infinite */
 
 #define E3D_SCHEME_ROE 0
 #define E3D_SCHEME_AUSM 1
@@ -84,6 +85,9 @@ typedef struct e3d_mesh_info {
 
 const char *e3d_last_error(void);
 const char *e3d_version(void);
+/* HIP devices visible to this process (0 without a device or driver). No counterpart in the
+ * reference, which runs on the CPU. */
+int e3d_device_count(int *n);
 
 /* ---- configuration (.e3d) ---------------------------------------------------------------- */
 int e3d_params_default(e3d_params *out);
@@ -182,6 +186,12 @@ int e3d_timestep(e3d_ctx *c, double *dt);
 int e3d_iterate(e3d_ctx *c, int n, double *rms_sum);
 /* force and moment coefficients of the current state (AeroCoefficients::Update) */
 int e3d_aero(e3d_ctx *c, double cf[3], double cm[3]);
+/* cp of every wall face, in wall-list order, as the last force update left it (AeroCoefficients::
+ * GetCp, include/solver/AeroCoefficients.hpp:53, written by EulerSolver::PrintCp,
+ * src/solver/EulerSolver.cpp:282-300): inside e3d_run that is the last sampled iteration, which
+ * need not be the last iteration. Zeros before any update. *n receives the wall face count;
+ * cp may be NULL to ask for it. */
+int e3d_wall_cp(e3d_ctx *c, double *cp, long long capacity, long long *n);
 /* Run loop with MIN_RESIDUAL, SAMPLING and coefficient convergence (EulerSolver.cpp:60-167).
  * rows: up to max_rows rows of 8 doubles (iter, rms, Cx, Cy, Cz, CMx, CMy, CMz) as residuals.dat
  * holds them. coeffs = CL, CD, CM after the loop. */
@@ -235,6 +245,12 @@ typedef struct e3d_batch_opts {
 	int use_graph;       /* replay iterations from a captured HIP graph */
 } e3d_batch_opts;
 
+/* how a batched case finished (the `done` value of e3d_batch_status / e3d_batch_poll) */
+#define E3D_CASE_RUNNING 0
+#define E3D_CASE_CONVERGED 1 /* residual drop or MIN_RESIDUAL reached */
+#define E3D_CASE_CAPPED 2    /* iteration cap reached first */
+#define E3D_CASE_FAILED 3    /* residual NaN or infinite: retired, the other cases are unaffected */
+
 typedef struct e3d_case_opts {
 	double drop;         /* converged when rms < drop * rms of the first iteration (0: off) */
 	int max_iter;        /* iteration cap */
@@ -246,7 +262,9 @@ int e3d_batch_create(int device, const e3d_batch_opts *opts, e3d_batch **out);
 /* device bytes a case of this preprocessed mesh holds while resident */
 int e3d_batch_case_bytes(const e3d_mesh *m, const e3d_case_opts *co, double *bytes);
 /* Admits a case (freestream start, FlowField::Initialize) and returns its id. The mesh may be
- * freed afterwards. E3D_ERR_BUSY when no slot is free or the case would exceed the budget. The
+ * freed afterwards. E3D_ERR_BUSY when no slot is free or the case does not fit in what the
+ * resident cases leave of the budget (retry after a removal); E3D_ERR_ARG when it needs more than
+ * the whole budget. The
  * upload overlaps the iterations already queued; the case starts with the next e3d_batch_run. May
  * be called from other threads while e3d_batch_run / e3d_batch_poll run (a batch is otherwise
  * used from one thread). */
@@ -255,14 +273,16 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 int e3d_batch_run(e3d_batch *b, int n);
 /* Waits for the queued iterations. */
 int e3d_batch_wait(e3d_batch *b);
-/* Waits, then reports a case's progress: done (converged or out of iterations), iterations,
- * RMS of the first and of the last iteration. */
+/* Waits, then reports a case's progress: done (E3D_CASE_*), iterations, RMS of the first and of
+ * the last iteration. A failed case fills the outputs and returns E3D_ERR_NUMERIC. */
 int e3d_batch_status(e3d_batch *b, int case_id, int *done, int *iters, double *rms0, double *rms_last);
-/* Waits, then lists the resident cases (up to cap) with their progress; *n receives their count. */
+/* Waits, then lists the resident cases (up to cap) with their progress (done = E3D_CASE_*); *n
+ * receives their count. */
 int e3d_batch_poll(e3d_batch *b, int *ids, int *done, int *iters, double *rms_last, int cap, int *n);
 /* Residual RMS history (one value per iteration, at most cap) and the force and moment
  * coefficients of the current state: coeffs = CL, CD, CM (AeroCoefficients::Update,
- * src/solver/AeroCoefficients.cpp:38-52, FP64). */
+ * src/solver/AeroCoefficients.cpp:38-52, FP64). A failed case fills the outputs and returns
+ * E3D_ERR_NUMERIC. */
 int e3d_batch_result(e3d_batch *b, int case_id, double *hist, int cap, int *n_hist, double coeffs[3], double cf[3],
                      double cm[3]);
 /* FP64 state of a case in the FlowField numbering as e3d_get_state gives it: interior cells from

@@ -16,9 +16,12 @@ sys.path.insert(0, str(ROOT / "oracle"))
 
 FIELDS = ("rho", "u", "v", "w", "p", "E", "H", "M")
 SMALL_CASES = ("block8", "naca9x9x5", "naca_small", "bump_small", "wing_small")
-# run-mode fixtures of the other integrators -> the fixture whose mesh they use
+# run-mode fixtures -> the fixture whose mesh they use: the other integrators, then Run's stopping
+# rules and sampling period (MIN_RESIDUAL, MIN_AEROCOEFF_ERROR, SAMPLING)
 INTEGRATOR_CASES = {"naca_euler": "naca_small", "naca_implicit": "naca_small", "bump_smooth": "bump_small",
-                    "wing_euler_smooth": "wing_small"}
+                    "wing_euler_smooth": "wing_small", "naca_minres": "naca_small", "naca_coeffconv": "naca_small",
+                    "bump_sampling7": "bump_small"}
+STOP_CASES = ("naca_minres", "naca_coeffconv", "bump_sampling7")
 MESH_KEYS_I = ("face2element", "face2nodeStart", "face2node", "element2faceStart", "element2face",
                "element2element", "wallGhost", "wallCell", "wallFace", "farGhost", "farCell", "farFace",
                "symGhost", "symCell", "symFace")
@@ -47,9 +50,11 @@ def golden_params(g: dict) -> dict:
     p = g["params"]
     ocl, ocd = axis_sign(int(p[9])), axis_sign(int(p[10]))
     ocm = (3 - ocl[0] - ocd[0], -ocl[1] * ocd[1])
+    stop = g["stop"] if "stop" in g else (1, 0.0, 0.0)   # SAMPLING, MIN_RESIDUAL, MIN_AEROCOEFF_ERROR
     return dict(mach=float(p[0]), aoa_deg=float(p[1]), gamma=float(p[2]), cfl=float(p[3]), sref=float(p[4]),
                 cref=float(p[5]), ref_point=(float(p[6]), float(p[7]), float(p[8])), orient=(ocl, ocd, ocm),
-                max_iter=int(p[11]), sampling=1, min_residual=0.0, min_aerocoeff_error=0.0, scheme=0)
+                max_iter=int(p[11]), sampling=int(stop[0]), min_residual=float(stop[1]),
+                min_aerocoeff_error=float(stop[2]), scheme=0)
 
 
 def integrator_params(g: dict) -> dict:

@@ -21,6 +21,13 @@ Cases:
   naca_implicit       naca_small mesh, TIME_INTEGRATION= 1 (implicit Euler: dt only), 5 iterations
   bump_smooth         bump_small mesh, RK5 with RESIDUAL_SMOOTHING= 1, 10 iterations
   wing_euler_smooth   wing_small mesh, explicit Euler with RESIDUAL_SMOOTHING= 1, 20 iterations
+  Run-mode cases of Run's stopping rules and sampling (RK5; the files also keep cp_part_0.dat):
+  naca_minres         naca_small mesh, SAMPLING= 7, MIN_RESIDUAL= 2.9e-5, MAX_ITER= 400: the residual
+                      test stops the run at iteration 197, between two samples
+  naca_coeffconv      naca_small mesh, SAMPLING= 4, MIN_AEROCOEFF_ERROR= 8.5e-4, MAX_ITER= 600: one
+                      isolated sample under the tolerance resets the counter, then two in a row stop
+                      the run
+  bump_sampling7      bump_small mesh, SAMPLING= 7, MAX_ITER= 40 (not a multiple of the period)
 
   Partitioned cases (the reference solver and harness on N MPI ranks, one dump per rank):
   block8_p4                   the reference's own 4-partition E3D_PRE output of the 8-hex block
@@ -79,8 +86,8 @@ SCHEME= 0
 TIME_INTEGRATION= {ti}
 RESIDUAL_SMOOTHING= {smooth}
 CFL= {cfl}
-MIN_RESIDUAL= 0
-MIN_AEROCOEFF_ERROR= 0
+MIN_RESIDUAL= {min_residual}
+MIN_AEROCOEFF_ERROR= {min_aero}
 MAX_ITER= {iters}
 SOLVER_LOG= {out}/solverlog.txt
 OUTPUT_FILE= {out}/out.dat
@@ -114,6 +121,11 @@ for _name, _base, _ti, _sm, _it in (("naca_euler", "naca_small", 0, 0, 150), ("n
                                      ("bump_smooth", "bump_small", 2, 1, 10),
                                      ("wing_euler_smooth", "wing_small", 0, 1, 20)):
     CASES[_name] = dict(CASES[_base], ti=_ti, smooth=_sm, iters=_it, seeds=(), run_mode=True, mesh_of=_base)
+for _name, _base, _it, _smp, _mr, _ma in (("naca_minres", "naca_small", 400, 7, 2.9e-5, 0.0),
+                                          ("naca_coeffconv", "naca_small", 600, 4, 0.0, 8.5e-4),
+                                          ("bump_sampling7", "bump_small", 40, 7, 0.0, 0.0)):
+    CASES[_name] = dict(CASES[_base], iters=_it, sampling=_smp, min_residual=_mr, min_aero=_ma, seeds=(),
+                        run_mode=True, mesh_of=_base)
 
 
 # Partitioned runs: the reference solver on N MPI ranks (mpiexec -np N) over N partition files.
@@ -202,13 +214,17 @@ def make_case(name: str, case: dict) -> None:
         cfg = td / "case.e3d"
         cfg.write_text(CONFIG.format(mesh=par.name, par=str(par), out=str(td / "out"), ocl=case["ocl"],
                                      ocd=case["ocd"], rx=rx, ry=ry, rz=rz, sref=case["sref"], cref=case["cref"],
-                                     sampling=1, mach=case["mach"], aoa=case["aoa"], cfl=case["cfl"],
-                                     iters=case["iters"], ti=case.get("ti", 2), smooth=case.get("smooth", 0)))
+                                     sampling=case.get("sampling", 1), mach=case["mach"], aoa=case["aoa"],
+                                     cfl=case["cfl"], iters=case["iters"], ti=case.get("ti", 2),
+                                     smooth=case.get("smooth", 0), min_residual=case.get("min_residual", 0.0),
+                                     min_aero=case.get("min_aero", 0.0)))
         data = {}
         params = np.array([case["mach"], case["aoa"], 1.4, case["cfl"], case["sref"], case["cref"], rx, ry, rz,
                            case["ocl"], case["ocd"], case["iters"]], dtype=np.float64)
         data["params"] = params
         data["integrator"] = np.array([case.get("ti", 2), case.get("smooth", 0)], dtype=np.int32)
+        if "sampling" in case:
+            data["stop"] = np.array([case["sampling"], case["min_residual"], case["min_aero"]], dtype=np.float64)
         if not case.get("run_only") and not case.get("run_mode"):
             mesh = run_harness("mesh", cfg, td / "mesh.txt")
             for k, v in mesh.items():
@@ -230,6 +246,8 @@ def make_case(name: str, case: dict) -> None:
         data["run_residuals_dat"] = parse_residuals(td / "out" / "residuals.dat")
         summary = (td / "out" / "summary.txt").read_text()
         data["run_summary_txt"] = np.frombuffer(summary.encode(), dtype=np.uint8)
+        if "sampling" in case:
+            data["run_cp_txt"] = np.frombuffer((td / "out" / "cp_part_0.dat").read_bytes(), dtype=np.uint8)
         data["par_sha256"] = np.frombuffer(hashlib.sha256(text.encode()).hexdigest().encode(), dtype=np.uint8)
         np.savez_compressed(HERE / f"{name}.npz", **data)
         if not case.get("run_only") and not case.get("run_mode"):
@@ -278,7 +296,8 @@ def make_part_case(name: str, case: dict) -> None:
         cfg.write_text(CONFIG.format(mesh="mesh.su2", par=str(td / "mesh_p#.par"), out=str(td / "out"), ocl=case["ocl"],
                                      ocd=case["ocd"], rx=rx, ry=ry, rz=rz, sref=case["sref"], cref=case["cref"],
                                      sampling=1, mach=case["mach"], aoa=case["aoa"], cfl=case["cfl"],
-                                     iters=case["iters"], ti=case.get("ti", 2), smooth=case.get("smooth", 0))
+                                     iters=case["iters"], ti=case.get("ti", 2), smooth=case.get("smooth", 0),
+                                     min_residual=0.0, min_aero=0.0)
                        .replace("PARTITION_FILES= 1", f"PARTITION_FILES= {n}"))
         data = {"params": np.array([case["mach"], case["aoa"], 1.4, case["cfl"], case["sref"], case["cref"], rx, ry, rz,
                                     case["ocl"], case["ocd"], case["iters"]], dtype=np.float64),

@@ -93,7 +93,7 @@ MIXED_SOL = ("             density          velocity_x          velocity_y      
 
 
 def config(d: Path, pattern: str, nparts: int) -> Path:
-    s = CONFIG.format(mesh="unused.su2", par=pattern, out=str(d), ocl=2, ocd=0, rx=0.0, ry=0.0, rz=0.0, sref=1.0,
+    s = CONFIG.format(min_residual=0, min_aero=0, mesh="unused.su2", par=pattern, out=str(d), ocl=2, ocd=0, rx=0.0, ry=0.0, rz=0.0, sref=1.0,
                       cref=1.0, sampling=1, mach=0.5, aoa=0.0, cfl=1.0, iters=1, ti=2, smooth=0)
     s = s.replace("PARTITION_FILES= 1", f"PARTITION_FILES= {nparts}")
     p = d / "case.e3d"

@@ -212,3 +212,52 @@ def test_server_results_equal_alone_results():
             alone = b.result(cid)
         assert r["iterations"] == st["iterations"]
         assert r["CL"] == alone["CL"] and r["CD"] == alone["CD"], (r, alone["CL"], alone["CD"])
+
+
+def test_nan_case_is_retired_with_an_error():
+    """A case whose residual turns NaN (here: a NaN Mach number through the ABI, so its freestream
+    and state are NaN) is retired with E3D_CASE_FAILED and E3D_ERR_NUMERIC from status and result,
+    while the other cases of the mix are bitwise what they are alone."""
+    import ctypes as C
+
+    from euler3d_amd import native as N
+
+    say("[gpu] NaN case in a mix")
+    a_mesh, c_mesh = naca(), naca(49, 25, thickness=0.09, camber=0.02)
+    pa, pc = naca_params(), naca_params(mach=0.6, aoa=2.0, scheme=1)
+    pbad = naca_params(mach=float("nan"))
+    alone_a, alone_c = mixed_run(a_mesh, pa, 60), mixed_run(c_mesh, pc, 60)
+    with Batch() as b:
+        ia = b.add(a_mesh, pa, drop=0.0, max_iter=60)
+        ib = b.add(naca(41, 21), pbad, drop=0.0, max_iter=60)
+        ic = b.add(c_mesh, pc, drop=0.0, max_iter=60)
+        b.run(60)
+        sb = b.status(ib)
+        rb = b.result(ib)
+        done, it = C.c_int(), C.c_int()
+        r0, r1 = C.c_double(), C.c_double()
+        rc = N.lib().e3d_batch_status(b._h, ib, C.byref(done), C.byref(it), C.byref(r0), C.byref(r1))
+        polled = {s["id"]: s for s in b.poll()}
+        ra, rcres = b.result(ia), b.result(ic)
+        sta, stc = b.get_state(ia), b.get_state(ic)
+    assert rc == N.E3D_ERR_NUMERIC and done.value == 3
+    assert sb["outcome"] == "failed" and sb["iterations"] == 1 and rb["failed"]
+    assert polled[ib]["outcome"] == "failed" and polled[ia]["outcome"] == "capped"
+    assert not ra["failed"] and not rcres["failed"]
+    for r, st, alone in ((ra, sta, alone_a), (rcres, stc, alone_c)):
+        assert bits_equal(r["rms"], alone["rms"])
+        for k in ("rho", "u", "v", "w", "p", "E", "H", "M"):
+            assert bits_equal(st[k], alone["state"][k]), k
+        assert r["CL"] == alone["CL"] and r["CD"] == alone["CD"]
+
+
+def test_case_larger_than_the_budget_is_refused_outright():
+    say("[gpu] oversized case")
+    m = naca(33, 17)
+    with Batch() as b:
+        b.add(m, naca_params(), drop=0.0, max_iter=100)
+        need = b.info()["bytes"]
+    with Batch(hbm_budget=0.5 * need) as b:
+        with pytest.raises(E3DError) as e:
+            b.add(m, naca_params(), drop=0.0, max_iter=100)
+        assert e.value.code == 1 and "whole HBM budget" in str(e.value)

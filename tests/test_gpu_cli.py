@@ -32,8 +32,11 @@ def summary_values(text: str) -> dict:
     return out
 
 
-@pytest.mark.parametrize("case", ["naca_small", "bump_small", "naca_euler", "wing_euler_smooth"])
+@pytest.mark.parametrize("case", ["naca_small", "bump_small", "naca_euler", "wing_euler_smooth", "naca_minres",
+                                  "naca_coeffconv", "bump_sampling7"])
 def test_cli_matches_reference_outputs(case, tmp_path):
+    """Also Run's stopping rules and SAMPLING > 1: cp_part_0.dat holds the cp of the last sampled
+    iteration, byte for byte as the reference's PrintCp writes it."""
     print(f"[gpu] E3D_Solver case={case}", flush=True)
     g = load_golden(case)
     base = INTEGRATOR_CASES.get(case, case)
@@ -44,9 +47,10 @@ def test_cli_matches_reference_outputs(case, tmp_path):
     cfg = tmp_path / "case.e3d"
     rx, ry, rz = pd["ref_point"]
     cfg.write_text(CONFIG.format(mesh=par.name, par=str(par), out=str(tmp_path / "out"), ocl=int(p[9]), ocd=int(p[10]),
-                                 rx=rx, ry=ry, rz=rz, sref=pd["sref"], cref=pd["cref"], sampling=1, mach=pd["mach"],
-                                 aoa=pd["aoa_deg"], cfl=pd["cfl"], iters=pd["max_iter"], ti=pd["time_integration"],
-                                 smooth=pd["residual_smoothing"]))
+                                 rx=rx, ry=ry, rz=rz, sref=pd["sref"], cref=pd["cref"], sampling=pd["sampling"],
+                                 mach=pd["mach"], aoa=pd["aoa_deg"], cfl=pd["cfl"], iters=pd["max_iter"],
+                                 ti=pd["time_integration"], smooth=pd["residual_smoothing"],
+                                 min_residual=pd["min_residual"], min_aero=pd["min_aerocoeff_error"]))
     r = subprocess.run([str(EXE), str(cfg)], cwd=str(tmp_path), capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
     # residuals.dat: header + iter rms Cx Cy Cz CMx CMy CMz, %.10e
@@ -69,6 +73,8 @@ def test_cli_matches_reference_outputs(case, tmp_path):
         expect = np.array([float("%e" % x) for x in g["run_" + k][:n]])
         np.testing.assert_array_equal(sol[:, col], expect, err_msg=k)
     assert (str(par) + ".bry") and (tmp_path / "out" / "cp_part_0.dat").exists()
+    if "run_cp_txt" in g:
+        assert (tmp_path / "out" / "cp_part_0.dat").read_bytes() == bytes(g["run_cp_txt"])
     # E3D_POST on the run: it reads <dir of OUTPUT_FILE>/<partition name>.sol, and its cell values
     # are the .sol numbers in the Tecplot block layout
     shutil.copyfile(str(par) + ".sol", tmp_path / "out" / (par.name + ".sol"))
@@ -102,7 +108,8 @@ def test_cli_partitioned_matches_reference_outputs(case, tmp_path):
     cfg.write_text(CONFIG.format(mesh="mesh.su2", par=str(tmp_path / "mesh_p#.par"), out=str(tmp_path / "out"),
                                  ocl=int(p[9]), ocd=int(p[10]), rx=rx, ry=ry, rz=rz, sref=pd["sref"], cref=pd["cref"],
                                  sampling=1, mach=pd["mach"], aoa=pd["aoa_deg"], cfl=pd["cfl"], iters=pd["max_iter"],
-                                 ti=pd["time_integration"], smooth=pd["residual_smoothing"])
+                                 ti=pd["time_integration"], smooth=pd["residual_smoothing"], min_residual=0.0,
+                                 min_aero=0.0)
                    .replace("PARTITION_FILES= 1", f"PARTITION_FILES= {n}"))
     r = subprocess.run([str(EXE), str(cfg)], cwd=str(tmp_path), capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]

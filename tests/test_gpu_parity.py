@@ -290,8 +290,11 @@ def test_gpu_error_paths(tmp_path):
 @pytest.mark.parametrize("path", ["fused", "unfused"])
 @pytest.mark.parametrize("case", sorted(INTEGRATOR_CASES))
 def test_gpu_other_integrators_match_reference(case, path, tmp_path, monkeypatch):
-    """Explicit Euler (TIME_INTEGRATION= 0), implicit Euler (1) and residual smoothing against full
-    reference runs: final state bit for bit, residual history, coefficients."""
+    """Explicit Euler (TIME_INTEGRATION= 0), implicit Euler (1), residual smoothing, and Run's
+    stopping rules and sampling period, against full reference runs: iteration count, final state
+    bit for bit, residual history, coefficients. The stop-rule thresholds sit at least 1e-3
+    relative away from every RMS of the run, far outside the 1e-12 by which the device's
+    fixed-tree RMS sum may differ from the reference's sequential sum."""
     say(f"[gpu] integrator case={case} path={path}")
     if path == "unfused":
         monkeypatch.setenv("E3D_STAGE_PATH", "unfused")

@@ -66,8 +66,10 @@ def test_oracle_run(case):
 
 @pytest.mark.parametrize("case", sorted(INTEGRATOR_CASES))
 def test_oracle_run_other_integrators(case):
-    """Explicit Euler (TIME_INTEGRATION= 0), implicit Euler (1: dt only) and implicit residual
-    smoothing (RESIDUAL_SMOOTHING= 1) against full reference runs."""
+    """Explicit Euler (TIME_INTEGRATION= 0), implicit Euler (1: dt only), implicit residual
+    smoothing (RESIDUAL_SMOOTHING= 1), and Run's stopping rules and sampling period (a MIN_RESIDUAL
+    crossed between two samples, MIN_AEROCOEFF_ERROR met twice in a row after an isolated hit,
+    SAMPLING= 7 with MAX_ITER not a multiple of it) against full reference runs."""
     g = load_golden(case)
     mesh = golden_mesh(load_golden(INTEGRATOR_CASES[case]))
     o = Oracle(mesh, integrator_params(g))
@@ -129,3 +131,26 @@ def test_oracle_threads_keep_every_bit():
     assert bits_equal(outs[0][0], outs[1][0]) and bits_equal(outs[0][2], outs[1][2])
     for k in FIELDS:
         assert bits_equal(outs[0][1][k], outs[1][1][k]), k
+
+
+def test_stop_rule_fixtures_reach_their_branches():
+    """The stop-rule fixtures exercise what they are for: naca_minres stops on the residual between
+    two samples (its last residuals.dat row is the break row, one iteration past a sample that is
+    not on the period); naca_coeffconv has an isolated sample under MIN_AEROCOEFF_ERROR that resets
+    the counter before two in a row stop the run; bump_sampling7 ends between two samples."""
+    g = load_golden("naca_minres")
+    rows, (period, min_res, _) = g["run_residuals_dat"], g["stop"]
+    it = int(g["run_iterations"][0])
+    assert rows[-1, 0] == it and it % int(period) != 0 and rows[-1, 1] < min_res <= rows[:-1, 1].min()
+    assert np.all(rows[:-1, 0] % period == 0)
+    g = load_golden("naca_coeffconv")
+    pd = golden_params(g)
+    rows, tol = g["run_residuals_dat"], pd["min_aerocoeff_error"]
+    (a0, s0), (a1, s1), (a2, s2) = pd["orient"]
+    c = np.stack([rows[:, 2 + a0] * s0, rows[:, 2 + a1] * s1, rows[:, 5 + a2] * s2], 1)
+    crit = (np.abs(np.diff(np.vstack([np.zeros(3), c]), axis=0)) < tol).all(1)
+    assert crit[-1] and crit[-2] and not crit[-3]
+    assert any(crit[k] and not crit[k + 1] for k in range(len(crit) - 3)), "no isolated hit before the stop"
+    assert int(g["run_iterations"][0]) < pd["max_iter"]
+    g = load_golden("bump_sampling7")
+    assert int(g["run_iterations"][0]) == 40 and g["run_residuals_dat"][-1, 0] == 35

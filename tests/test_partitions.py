@@ -371,7 +371,7 @@ def test_e3d_pre_executable(tmp_path):
         d = tmp_path / f"n{n}"
         d.mkdir()
         cfg = d / "case.e3d"
-        cfg.write_text(CONFIG.format(mesh=str(src), par=str(d / "part_#.par"), out=str(d), ocl=2, ocd=0, rx=0.0, ry=0.0,
+        cfg.write_text(CONFIG.format(min_residual=0, min_aero=0, mesh=str(src), par=str(d / "part_#.par"), out=str(d), ocl=2, ocd=0, rx=0.0, ry=0.0,
                                      rz=0.0, sref=1.0, cref=1.0, sampling=1, mach=0.5, aoa=0.0, cfl=1.0, iters=1, ti=2,
                                      smooth=0).replace("PARTITION_FILES= 1", f"PARTITION_FILES= {n}"))
         r = subprocess.run([str(ROOT / "euler3d_amd" / "bin" / "E3D_PRE"), str(cfg)], capture_output=True, text=True,

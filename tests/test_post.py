@@ -37,7 +37,7 @@ def stage(case: str, d: Path) -> Path:
     if case == "bump_small":
         (d / "bump_small.par").write_bytes(gzip.decompress((GOLDEN / "bump_small.par.gz").read_bytes()))
     pattern, n = CASES[case]
-    s = CONFIG.format(mesh="unused.su2", par=str(d / pattern), out=str(d), ocl=2, ocd=0, rx=0.0, ry=0.0, rz=0.0,
+    s = CONFIG.format(min_residual=0, min_aero=0, mesh="unused.su2", par=str(d / pattern), out=str(d), ocl=2, ocd=0, rx=0.0, ry=0.0, rz=0.0,
                       sref=1.0, cref=1.0, sampling=1, mach=0.5, aoa=0.0, cfl=1.0, iters=1, ti=2, smooth=0)
     cfg = d / "case.e3d"
     cfg.write_text(s.replace("PARTITION_FILES= 1", f"PARTITION_FILES= {n}"))
