@@ -134,10 +134,16 @@ class Server:
                 # retry the waiting requests once the admissions in flight have settled, after a
                 # retirement freed room, or when nothing is resident at all
                 if waiting and in_flight == 0 and (freed or empty):
-                    retry, waiting[:] = list(waiting), []
-                    freed = False
-                    for j in retry:
-                        futures.append(pool.submit(admit, j, empty))
+                    if empty:
+                        # into an empty engine a request must fit, or it never will: try the first
+                        # alone, the others once it has settled
+                        futures.append(pool.submit(admit, waiting.pop(0), True))
+                        freed = True
+                    else:
+                        retry, waiting[:] = list(waiting), []
+                        freed = False
+                        for j in retry:
+                            futures.append(pool.submit(admit, j))
         ft.join()
         pool.shutdown()
         return rec

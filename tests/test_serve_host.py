@@ -92,12 +92,13 @@ def test_every_request_ends_with_an_outcome():
 
 
 @pytest.mark.timeout(60)
+@pytest.mark.parametrize("seed", range(6))
 @pytest.mark.parametrize("delay", [0.0, 0.003])
-def test_waiting_requests_are_backfilled(delay):
+def test_waiting_requests_are_backfilled(delay, seed):
     """More requests than slots and budget at once, with slow admissions (so a request can be
     refused after the last retirement took its retry list): all are served, never more resident
     than fit."""
-    rng = np.random.default_rng(5)
+    rng = np.random.default_rng(seed)
     reqs = [dict(bytes=float(rng.uniform(1.0, 4.0)), iters=int(rng.integers(5, 80))) for _ in range(30)]
     rec, eng = serve(reqs, np.zeros(30), budget=8.0, slots=3, delay=delay)
     assert all(r["outcome"] == "converged" for r in rec)
