@@ -147,14 +147,14 @@ void initFreestream(e3d_ctx *c) {
 	c->wMode = false;
 }
 
-void runStage(e3d_ctx *c, int stage, double *rmsPartial) {
+void runStage(e3d_ctx *c, int stage, double *rmsPartial, bool timed) {
 	static const double alphas[5] = {0.0533, 0.1263, 0.2375, 0.4414, 1.0};
 	auto s = c->state();
 	hipEvent_t evA = nullptr, evB = nullptr;
 	auto record = [&](hipEvent_t e) {
 		if (e) hipCheck(hipEventRecord(e, c->stream), "hipEventRecord");
 	};
-	if (c->timing) {
+	if (c->timing && timed) {
 		if (c->evUsed + 2 > c->evPool.size()) {
 			for (int i = 0; i < 64; i++) {
 				hipEvent_t e;
@@ -231,12 +231,13 @@ void toReference(const e3d_ctx *c, const std::vector<double> &dev, int k, double
 	for (int i = c->nElem; i < c->nTot; i++) ref[i] = dev[(size_t) i * e3d::kStateWidth + k];
 }
 
-// one iteration of Run's body; the sum of R_rho^2 of the last stage lands in *dsum (device)
-void runIteration(e3d_ctx *c, double *dsum) {
+// one iteration of Run's body; the sum of R_rho^2 of the last stage lands in *dsum (device).
+// timed = false leaves out the per-launch timing events (they are created on demand).
+void runIteration(e3d_ctx *c, double *dsum, bool timed = true) {
 	if (c->prm.time_integration == 2) {
-		for (int stage = 0; stage < 5; stage++) runStage(c, stage, stage == 4 ? c->partial.p : nullptr);
+		for (int stage = 0; stage < 5; stage++) runStage(c, stage, stage == 4 ? c->partial.p : nullptr, timed);
 	} else {
-		runStage(c, 0, c->partial.p); // explicit Euler step, or (implicit) residual only
+		runStage(c, 0, c->partial.p, timed); // explicit Euler step, or (implicit) residual only
 	}
 	e3d::launchSum(c->partial.p, c->partialCount, dsum, c->stream);
 }
@@ -636,6 +637,8 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		if (device < 0 || device >= nDev) return fail(E3D_ERR_DEVICE, "no HIP device " + std::to_string(device));
 		c->device = device;
 		hipCheck(hipSetDevice(device), "hipSetDevice");
+		// address of the allocation-free iteration's static scratch on this device (once per device)
+		hipCheck(e3d::iterScratchInit(device), "hipGetSymbolAddress(scratch)");
 		hipCheck(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
 		hipCheck(hipEventCreate(&c->ev0), "hipEventCreate");
 		hipCheck(hipEventCreate(&c->ev1), "hipEventCreate");
@@ -926,6 +929,31 @@ int e3d_iterate(e3d_ctx *c, int n, double *rms_sum) {
 		c->lastMs = ms;
 		if (rms_sum && n > 0)
 			hipCheck(hipMemcpy(rms_sum, sums.p, sizeof(double) * n, hipMemcpyDeviceToHost), "D2H");
+		return E3D_OK;
+	});
+}
+
+int e3d_iterate_noalloc(e3d_ctx *c, int n, double *rms_sum) {
+	if (!c || n < 0) return fail(E3D_ERR_ARG, "bad argument");
+	if (n > e3d::kIterScratchMax)
+		return fail(E3D_ERR_ARG, "e3d_iterate_noalloc: " + std::to_string(n) + " iterations exceed the " +
+		                             std::to_string(e3d::kIterScratchMax) + " its static scratch holds");
+	return guarded([&] {
+		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		double *sums = e3d::iterScratch(c->device);
+		if (!sums) return fail(E3D_ERR_DEVICE, "static scratch not set up on this device");
+		hipCheck(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
+		// every iteration writes its own sum: nothing in the scratch needs clearing first
+		for (int it = 0; it < n; it++) runIteration(c, sums + it, false);
+		hipCheck(hipEventRecord(c->ev1, c->stream), "hipEventRecord");
+		c->check();
+		c->sync();
+		checkSmoothing(c);
+		float ms = 0;
+		hipCheck(hipEventElapsedTime(&ms, c->ev0, c->ev1), "hipEventElapsedTime");
+		c->lastMs = ms;
+		if (rms_sum && n > 0)
+			hipCheck(hipMemcpy(rms_sum, sums, sizeof(double) * n, hipMemcpyDeviceToHost), "D2H");
 		return E3D_OK;
 	});
 }

@@ -132,6 +132,12 @@ void launchMathCheck(unsigned long long seed, long long n, unsigned long long *b
 // interior primitive records of s from the conservative states W (the stage's expressions), with
 // M = c / |u|
 void launchWToPrim(const DevMesh &m, const double *W, DevState s, const DevFree &q, hipStream_t st);
+// Static per-device scratch of the allocation-free iteration entry point: kIterScratchMax doubles
+// (512 KiB, one RMS sum per iteration). iterScratchInit resolves its address on `device`, which
+// must be current, once per device; iterScratch returns that address (null before the init).
+constexpr int kIterScratchMax = 65536;
+hipError_t iterScratchInit(int device);
+double *iterScratch(int device);
 // deterministic sum of n doubles into out[0] (single workgroup, fixed order)
 void launchSum(const double *in, int n, double *out, hipStream_t st);
 // interior Mach number from the primitives (FlowField.cpp:157), for state read-back

@@ -23,6 +23,7 @@
 #include "device.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace e3d {
 
@@ -1258,6 +1259,36 @@ void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState 
 #undef E3D_LAUNCH_K
 #undef E3D_LAUNCH
 #undef E3D_LAUNCH_B
+}
+
+// Per-iteration RMS sums of the allocation-free iteration entry point: a static device array,
+// one copy per device, shared by every call on that device. Host code never passes `scratch`
+// itself to a kernel (in host code the name is only a stand-in); it uses the address that
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double scratch[kIterScratchMax];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMu;
+double *gScratch[kMaxDevices] = {};
+}// namespace
+
+hipError_t iterScratchInit(int device) {
+	if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+	std::lock_guard<std::mutex> lk(gScratchMu);
+	if (gScratch[device]) return hipSuccess;
+	// the caller has made `device` current
+	void *ptr = nullptr;
+	hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+	if (e != hipSuccess) return e;
+	gScratch[device] = static_cast<double *>(ptr);
+	return hipSuccess;
+}
+
+double *iterScratch(int device) {
+	if (device < 0 || device >= kMaxDevices) return nullptr;
+	std::lock_guard<std::mutex> lk(gScratchMu);
+	return gScratch[device];
 }
 
 void launchSum(const double *in, int n, double *out, hipStream_t st) {

@@ -99,6 +99,7 @@ _SIGS = {
     "e3d_residual": (C.c_int, [P, DP]),
     "e3d_timestep": (C.c_int, [P, DP]),
     "e3d_iterate": (C.c_int, [P, C.c_int, DP]),
+    "e3d_iterate_noalloc": (C.c_int, [P, C.c_int, DP]),
     "e3d_aero": (C.c_int, [P, DP, DP]),
     "e3d_wall_cp": (C.c_int, [P, DP, C.c_longlong, C.POINTER(C.c_longlong)]),
     "e3d_run": (C.c_int, [P, DP, C.c_int, IP, IP, DP, DP, DP]),

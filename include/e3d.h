@@ -184,6 +184,13 @@ int e3d_timestep(e3d_ctx *c, double *dt);
  * or NULL) receives sum over interior cells of R_rho^2 of each iteration's last stage
  * (EulerSolver.cpp:96); RMS = sqrt(rms_sum / n_elem). */
 int e3d_iterate(e3d_ctx *c, int n, double *rms_sum);
+/* e3d_iterate without any device allocation: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The per-iteration sums go through a static device array of 65536 doubles
+ * (512 KiB per device, held until the process ends) instead of a buffer allocated per call, so n
+ * is at most 65536 (E3D_ERR_ARG beyond). Every context on a device shares that array: calls that
+ * overlap on one device (from several threads) race. Results are bitwise those of e3d_iterate. */
+int e3d_iterate_noalloc(e3d_ctx *c, int n, double *rms_sum);
 /* force and moment coefficients of the current state (AeroCoefficients::Update) */
 int e3d_aero(e3d_ctx *c, double cf[3], double cm[3]);
 /* cp of every wall face, in wall-list order, as the last force update left it (AeroCoefficients::
