@@ -715,7 +715,13 @@ __device__ __forceinline__ int logicalBlock(int w, int nBlocks, int xcds, bool r
 // the interior state is read from and written to (0: primitive records in and out, 1: primitive
 // records in and conservative states out, 2: conservative states in and out); ghost records are
 // read from `in` and written to `out` in every layout.
-template<int SCHEME, bool NT, bool MK, int INTEG, int LAYOUT, int CB>
+// LDS strides of the hot configuration (256-cell blocks, conservative states in and out, halo
+// and slot counts that fit): per-cell rows of kFixCap doubles and flux rows of kFixSlots. As
+// compile-time constants every LDS address of a row is an immediate offset from one lane address.
+// Both are odd, so the rows of one cell (and the five fluxes of one slot) sit in different banks.
+constexpr int kFixCap = 441, kFixSlots = 637;
+
+template<int SCHEME, bool NT, bool MK, int INTEG, int LAYOUT, int CB, bool FIXED = false>
 __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out,
                                                       const double *Win, double *Wout, DevFree q, double *W0,
                                                       double alpha, int firstStage, int machDerived,
@@ -727,8 +733,8 @@ __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState
 	const int nc = min(CB, m.nElem - c0);
 	const int s0 = p.blockSlot[b], ns = p.blockSlot[b + 1] - s0;
 	const int h0 = p.blockHalo[b], nh = p.blockHalo[b + 1] - h0;
-	const int cap = CB + p.maxHalo;
-	const int ms = p.maxSlots;
+	const int cap = FIXED ? kFixCap : CB + p.maxHalo;
+	const int ms = FIXED ? kFixSlots : p.maxSlots;
 	// LDS: [6][cap] rho, u, v, w, p, H of own cells then halo cells; with LAYOUT 2 two more rows
 	// [cap], sqrt(rho) and the sound speed sqrt(gamma p / rho) as the fluxes and the time step form
 	// them, else [2][CB] E, M of own cells; then [5][ms] F*S per slot (first the own W of LAYOUT 2)
@@ -1141,7 +1147,13 @@ inline int blocksFor(int n) { return (n + kBlock - 1) / kBlock; }
 
 int stageBlocks(int nElem) { return blocksFor(nElem); }
 
+// whether a plan runs the fixed-stride kernel (256-cell blocks, conservative-state layout)
+bool stageFusedFixed(int cellsPerBlock, int maxSlots, int maxHalo, int layout) {
+	return cellsPerBlock == 256 && layout == 2 && cellsPerBlock + maxHalo <= kFixCap && maxSlots <= kFixSlots;
+}
+
 size_t stageFusedLdsBytes(int cellsPerBlock, int maxSlots, int maxHalo, int layout) {
+	if (stageFusedFixed(cellsPerBlock, maxSlots, maxHalo, layout)) maxHalo = kFixCap - cellsPerBlock, maxSlots = kFixSlots;
 	// the flux region also stages the block's conservative states (5 per cell) before the fluxes
 	const size_t cb = (size_t) cellsPerBlock, cap = cb + maxHalo;
 	const size_t flux = 5 * std::max((size_t) maxSlots, cb);
@@ -1228,12 +1240,16 @@ void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState 
 	if (p.nBlocks == 0) return;
 	const int rev = t.alternate ? (parity & 1) : 0;
 	const size_t lds = stageFusedLdsBytes(p.cellsPerBlock, p.maxSlots, p.maxHalo, integ == 1 ? 0 : layout);
+	const bool fixed = integ != 1 && stageFusedFixed(p.cellsPerBlock, p.maxSlots, p.maxHalo, layout);
 	const int x = t.xcds;
 #define E3D_LAUNCH_B(S, N, K, B)                                                                               \
 	do {                                                                                                       \
 		if (integ == 1)                                                                                        \
 			hipLaunchKernelGGL((kStageFused<S, N, K, 1, 0, B>), dim3(p.nBlocks), dim3(B), lds, st, m, p, in, out, \
 			                   Win, Wout, q, W0, alpha, 0, machDerived, rmsPartial, x, rev);                      \
+		else if (layout == 2 && fixed)                                                                         \
+			hipLaunchKernelGGL((kStageFused<S, N, K, 0, 2, 256, true>), dim3(p.nBlocks), dim3(256), lds, st, m, p, \
+			                   in, out, Win, Wout, q, W0, alpha, firstStage, machDerived, rmsPartial, x, rev);     \
 		else if (layout == 2)                                                                                  \
 			hipLaunchKernelGGL((kStageFused<S, N, K, 0, 2, B>), dim3(p.nBlocks), dim3(B), lds, st, m, p, in, out, \
 			                   Win, Wout, q, W0, alpha, firstStage, machDerived, rmsPartial, x, rev);             \
