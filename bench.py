@@ -11,7 +11,11 @@ Prints one JSON line. value = cell-stage updates per second = cells * 5 * steps 
 roofline: algorithmic bytes of one stage (152 + 32 * faces/cells B per cell-stage, FP64) divided
 by the mean duration of the stage kernel(s), measured with HIP events on the solver's stream
 during the timed steps, against 8 TB/s. cpu_baseline: the CPU oracle (single-thread C
-restatement of the reference's arithmetic) timed on one RK5 iteration of the same mesh.
+restatement of the reference's arithmetic) timed on one RK5 iteration of the same mesh, with its
+recorded speed ratio to the compiled reference (oracle/cpu_anchor.json) and its timings on the
+NACA and wing meshes. mixed_precision: the same workload on the FP32-flux / FP64-update engine.
+served: a bounded Poisson stream of six requests, served by continuous batching against the plain
+FP64 path one request at a time.
 """
 from __future__ import annotations
 
@@ -126,11 +130,55 @@ def cpu_baseline(mesh, params: dict, n_cells: int) -> dict:
         dtm = (time.perf_counter() - t0) / 2
     finally:
         oracle_ctypes.set_threads(1)
-    return {"value": n_cells * 5 / dt, "unit": "cell-stage/s", "cores": 1, "kind": "port",
-            "sample": f"1 RK5 iteration (5 stages) of the full {n_cells}-cell mesh, {dt:.1f} s, oracle C "
-                      f"restatement compiled -O2, single thread, {cpu_model()}",
-            "all_cores": {"value": n_cells * 5 / dtm, "unit": "cell-stage/s", "cores": cores, "kind": "port",
-                          "sample": f"2 RK5 iterations, {dtm:.2f} s each, OpenMP over {cores} threads"}}
+    out = {"value": n_cells * 5 / dt, "unit": "cell-stage/s", "cores": 1, "kind": "port",
+           "sample": f"1 RK5 iteration (5 stages) of the full {n_cells}-cell mesh, {dt:.1f} s, oracle C "
+                     f"restatement compiled -O2, single thread, {cpu_model()}",
+           "all_cores": {"value": n_cells * 5 / dtm, "unit": "cell-stage/s", "cores": cores, "kind": "port",
+                         "sample": f"2 RK5 iterations, {dtm:.2f} s each, OpenMP over {cores} threads"}}
+    anchor = ROOT / "oracle" / "cpu_anchor.json"
+    if anchor.exists():
+        # the port against the compiled reference, both single-core on one host (recorded where the
+        # reference builds): the reference's speed on this host is estimated by scaling the port's
+        rec = json.loads(anchor.read_text())
+        ratio = rec["port_over_reference_time"]
+        out["anchor_vs_reference"] = {
+            "port_over_reference_time": ratio, "reference_equivalent_value": out["value"] * ratio,
+            "unit": "cell-stage/s", "case": rec["case"], "host": rec["host"],
+            "reference_s_per_iteration": rec["reference"]["s_per_iteration"],
+            "port_s_per_iteration": rec["port"]["s_per_iteration"], "reference_build": rec["reference"]["build"]}
+    out["other_workloads"] = [oracle_timing(*w) for w in other_cpu_workloads()]
+    return out
+
+
+def other_cpu_workloads():
+    """The plain path's two other meshes for the single-thread oracle timing: the 129 x 129 x 2 NACA
+    O-grid (16,384 hexes, 20 iterations) and the 193 x 97 x 65 wing (1,179,648 hexes, 1 iteration)."""
+    from euler3d_amd import Mesh
+
+    naca = lambda: Mesh.naca(ni=129, nj=129, nk=2, thickness=0.12, wall_spacing=1e-3,  # noqa: E731
+                             farfield_radius=20.0, dz=0.1)
+    wing = lambda: Mesh.wing(ni=193, nj=97, nk=65, thickness=0.10, sweep_deg=30.0, taper=0.56,  # noqa: E731
+                             semispan=1.5, farfield=15.0)
+    return [("NACA0012 O-grid 129x129x2, M 0.8, AoA 1.25 deg, Roe, RK5", naca, dict(mach=0.8, aoa_deg=1.25), 20),
+            ("swept wing O-H 193x97x65, M 0.84, AoA 3 deg, Roe, RK5", wing, dict(mach=0.84, aoa_deg=3.0), 1)]
+
+
+def oracle_timing(name, make, params, iters) -> dict:
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from oracle_ctypes import Oracle
+
+    m = make().preprocess(1.0)
+    n = m.info()["n_elem"]
+    names = ("face2element", "element2faceStart", "element2face", "faceNormals", "faceNormalsUnit",
+             "faceSurfaces", "faceCenters", "cellVolumes", "wallGhost", "wallCell", "wallFace", "farGhost",
+             "farCell", "farFace", "symGhost", "symCell", "symFace")
+    o = Oracle({k: m.get(k) for k in names}, dict(params, cfl=1.5))
+    t0 = time.perf_counter()
+    o.iterate(iters)
+    dt = time.perf_counter() - t0
+    m.close()
+    return {"workload": name, "cells": n, "iterations": iters, "value": n * 5 * iters / dt, "unit": "cell-stage/s",
+            "cores": 1, "kind": "port", "s_per_iteration": dt / iters}
 
 
 def build_meshes(cases, workers: int = 8):
@@ -248,6 +296,43 @@ def bench_serve(a) -> list[dict]:
     return lines
 
 
+def served_leg(a) -> dict:
+    """A bounded served stream under the same clock as the default run: six requests of the three
+    mesh families (10k-133k cells, seed 0), Poisson arrivals at 20 req/s, each to a 1e-6 residual
+    drop or 5000 iterations. The continuous-batching server on the mixed-precision engine against
+    the plain FP64 path serving one request at a time in arrival order (its latencies follow from
+    its measured service times); meshes are built before either clock starts."""
+    from euler3d_amd.serve import Server, fifo_latencies, percentile, plain_solve, summarize
+    from euler3d_amd.workloads import poisson_arrivals, request_mix
+
+    cases = request_mix(6, seed=0, min_cells=1e4, max_cells=2e5)
+    built = build_meshes(cases)
+    cells = [m.info()["n_elem"] for m, _ in built]
+    plain = [plain_solve(m, p, drop=1e-6, max_iter=5000) for m, p in built]
+    arr = poisson_arrivals(len(cases), 20.0, seed=0)
+    srv = Server(drop=1e-6, max_iter=5000, hbm_budget=200e9, admit_workers=4)
+    try:
+        rec = srv.serve([(m, p, c.kind) for (m, p), c in zip(built, cases)], arr)
+    finally:
+        srv.close()
+    for m, _ in built:
+        m.close()
+    sv = summarize(rec)
+    lat, span = fifo_latencies(arr, [q["service"] for q in plain])
+    outcomes = lambda rs: {k: sum(1 for r in rs if r["outcome"] == k) for k in ("converged", "capped", "failed")}  # noqa: E731
+    return {"metric": "served solves/h", "value": sv["solves_per_h"], "unit": "solves/h",
+            "workload": f"6 requests (naca/bump/wing, {min(cells)}-{max(cells)} cells, seed 0), Poisson 20 req/s, "
+                        "1e-6 drop or 5000 iterations, 200 GB HBM budget",
+            "served": {"p50_s": sv["p50_s"], "p99_s": sv["p99_s"], "makespan_s": sv["makespan_s"],
+                       "solves_per_h": sv["solves_per_h"], **outcomes(rec)},
+            "plain_fp64": {"p50_s": percentile(lat, 50), "p99_s": percentile(lat, 99), "makespan_s": span,
+                           "solves_per_h": 3600.0 * len(cases) / span, **outcomes(plain)},
+            "vs_plain_solves_per_h": sv["solves_per_h"] / (3600.0 * len(cases) / span),
+            "max_abs_dCL_vs_fp64": max(abs(r["CL"] - q["CL"]) for r, q in zip(rec, plain)),
+            "max_abs_dCD_vs_fp64": max(abs(r["CD"] - q["CD"]) for r, q in zip(rec, plain)),
+            "iterations": [[r["iterations"], q["iterations"]] for r, q in zip(rec, plain)]}
+
+
 def bench_wing(a) -> dict:
     """The swept tapered wing of the accuracy case, 193x97x65 nodes (1,179,648 hexes), M 0.84,
     AoA 3 deg: the FP64 path and the mixed-precision engine (FP32 flux and BC, FP64 update with
@@ -309,6 +394,7 @@ def main() -> None:
     ap.add_argument("--scheme", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the bounded served leg")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run one plain device-to-device copy of this many MiB (a known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -341,14 +427,13 @@ def main() -> None:
     from euler3d_amd import EulerSolver, Mesh, default_params
 
     if a.copy_calib_mib:
-        import torch
+        import ctypes
 
-        src = torch.ones(a.copy_calib_mib << 20, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copy calibration: {a.copy_calib_mib} MiB read + {a.copy_calib_mib} MiB written")
-        del src, dst
+        from euler3d_amd import native
+
+        ms = ctypes.c_double()
+        native.check(native.lib().e3d_copy_probe(0, a.copy_calib_mib << 20, ctypes.byref(ms)))
+        log(f"copy calibration: {a.copy_calib_mib} MiB read + {a.copy_calib_mib} MiB written in {ms.value:.3f} ms")
 
     t0 = time.perf_counter()
     mesh = Mesh.bump(nx=a.nx, ny=a.ny, nz=a.nz, jitter=0.1, seed=0, permute=True)
@@ -401,6 +486,10 @@ def main() -> None:
         out["mixed_precision"] = mixed_precision_leg(mesh, prm, n_cells, n_faces, a.steps, a.warmup)
         log(f"mixed precision: {out['mixed_precision']['value']:.3e} cell-stage/s, kMixStage "
             f"{out['mixed_precision']['roofline']['kernel_ms'] * 1e3:.1f} us")
+    if not a.no_serve:
+        out["served"] = served_leg(a)
+        log(f"served leg: {out['served']['value']:.0f} solves/h ({out['served']['vs_plain_solves_per_h']:.2f}x plain), "
+            f"p50 {out['served']['served']['p50_s']:.2f} s p99 {out['served']['served']['p99_s']:.2f} s")
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, pd, n_cells)
         log(f"cpu baseline: {out['cpu_baseline']['value']:.3e} cell-stage/s")

@@ -1173,6 +1173,37 @@ int e3d_math_selfcheck(int device, long long n, unsigned long long seed, unsigne
 	});
 }
 
+int e3d_copy_probe(int device, long long bytes, double *ms) {
+	if (bytes <= 0 || bytes % 16) return fail(E3D_ERR_ARG, "copy probe: bytes must be a positive multiple of 16");
+	return guarded([&] {
+		hipCheck(hipSetDevice(device), "hipSetDevice");
+		void *src = nullptr, *dst = nullptr;
+		hipEvent_t a = nullptr, z = nullptr;
+		hipError_t e = hipMalloc(&src, (size_t) bytes);
+		if (e == hipSuccess) e = hipMalloc(&dst, (size_t) bytes);
+		if (e == hipSuccess) e = hipMemset(src, 1, (size_t) bytes);
+		if (e == hipSuccess) e = hipEventCreate(&a);
+		if (e == hipSuccess) e = hipEventCreate(&z);
+		if (e == hipSuccess) e = hipDeviceSynchronize();
+		if (e == hipSuccess) e = hipEventRecord(a, nullptr);
+		if (e == hipSuccess) {
+			e3d::launchCopy16(src, dst, bytes / 16, nullptr);
+			e = hipGetLastError();
+		}
+		if (e == hipSuccess) e = hipEventRecord(z, nullptr);
+		if (e == hipSuccess) e = hipEventSynchronize(z);
+		float t = 0;
+		if (e == hipSuccess) e = hipEventElapsedTime(&t, a, z);
+		if (a) (void) hipEventDestroy(a);
+		if (z) (void) hipEventDestroy(z);
+		if (src) (void) hipFree(src);
+		if (dst) (void) hipFree(dst);
+		hipCheck(e, "copy probe");
+		if (ms) *ms = t;
+		return E3D_OK;
+	});
+}
+
 int e3d_last_elapsed_ms(e3d_ctx *c, double *ms) {
 	if (!c || !ms) return fail(E3D_ERR_ARG, "null argument");
 	*ms = c->lastMs;

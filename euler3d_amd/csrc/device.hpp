@@ -129,6 +129,8 @@ void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState 
 // bitwise comparison of the short square root, reciprocal and quotient with the library's on n
 // pseudo-random inputs; bad[0..2] (device, zeroed by the caller) count the mismatches
 void launchMathCheck(unsigned long long seed, long long n, unsigned long long *bad, hipStream_t st);
+// dst[i] = src[i] for `chunks` 16 B chunks (grid-stride; both 16 B aligned)
+void launchCopy16(const void *src, void *dst, long long chunks, hipStream_t st);
 // interior primitive records of s from the conservative states W (the stage's expressions), with
 // M = c / |u|
 void launchWToPrim(const DevMesh &m, const double *W, DevState s, const DevFree &q, hipStream_t st);

@@ -1141,6 +1141,13 @@ __global__ void __launch_bounds__(kBlock) kAero(int nWall, const int *wallCell, 
 	}
 }
 
+// plain copy of n 16 B chunks, each read once and written once (a kernel of known traffic for
+// scaling the memory counters)
+__global__ void __launch_bounds__(kBlock) kCopy16(const double2 *src, double2 *dst, long long n) {
+	const long long stride = (long long) gridDim.x * kBlock;
+	for (long long i = (long long) blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 inline int blocksFor(int n) { return (n + kBlock - 1) / kBlock; }
 
 }// namespace
@@ -1175,6 +1182,14 @@ int stageFusedWavesPerCU(int cellsPerBlock, int maxSlots, int maxHalo, int schem
 		                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kStageFused<0, true, true, 0, 2, 256>, 256, lds);
 	if (e != hipSuccess || n <= 0) n = 1;
 	return n * cellsPerBlock / 64;
+}
+
+void launchCopy16(const void *src, void *dst, long long chunks, hipStream_t st) {
+	if (chunks <= 0) return;
+	const long long want = (chunks + kBlock - 1) / kBlock;
+	const unsigned grid = (unsigned) std::min<long long>(want, 65536);
+	hipLaunchKernelGGL(kCopy16, dim3(grid), dim3(kBlock), 0, st, static_cast<const double2 *>(src),
+	                   static_cast<double2 *>(dst), chunks);
 }
 
 void launchMathCheck(unsigned long long seed, long long n, unsigned long long *bad, hipStream_t st) {

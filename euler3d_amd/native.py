@@ -71,6 +71,7 @@ _SIGS = {
     "e3d_version": (C.c_char_p, []),
     "e3d_device_count": (C.c_int, [IP]),
     "e3d_math_selfcheck": (C.c_int, [C.c_int, C.c_longlong, C.c_ulonglong, C.POINTER(C.c_ulonglong)]),
+    "e3d_copy_probe": (C.c_int, [C.c_int, C.c_longlong, DP]),
     "e3d_params_default": (C.c_int, [C.POINTER(Params)]),
     "e3d_config_read": (C.c_int, [C.c_char_p, C.POINTER(Params), C.c_char_p, C.c_int, C.c_char_p, C.c_int]),
     "e3d_config_partition": (C.c_int, [C.c_char_p, C.c_int, C.c_char_p, C.c_int, IP]),

@@ -227,6 +227,10 @@ int e3d_kernel_timing(e3d_ctx *c, double *avg_ms, int *launches, char *name, int
  * the library results. mismatches[0..2] receive the counts for the square root, the reciprocal and
  * the quotient. No counterpart in the reference (a check of this library's arithmetic). */
 int e3d_math_selfcheck(int device, long long n, unsigned long long seed, unsigned long long mismatches[3]);
+/* One launch of a plain copy kernel over `bytes` (a positive multiple of 16) on device `device`:
+ * every byte read once and written once, so its memory-counter readings scale the stage kernels'.
+ * *ms (may be NULL) receives its duration. No counterpart in the reference (measurement hook). */
+int e3d_copy_probe(int device, long long bytes, double *ms);
 void e3d_destroy(e3d_ctx *c);
 
 /* ---- mixed-precision multi-case engine (batched and served solves) ------------------------
