@@ -721,8 +721,11 @@ __device__ __forceinline__ int logicalBlock(int w, int nBlocks, int xcds, bool r
 // Both are odd, so the rows of one cell (and the five fluxes of one slot) sit in different banks.
 constexpr int kFixCap = 441, kFixSlots = 637;
 
-template<int SCHEME, bool NT, bool MK, int INTEG, int LAYOUT, int CB, bool FIXED = false>
-__global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out,
+// NTH threads serve the CB cells of a block (NTH = CB, or 2 CB where a block's LDS leaves room for
+// one workgroup per CU only: more waves then share the same LDS); threads t >= CB take part in the
+// loads and the flux loop, and the gather and update run on the first CB.
+template<int SCHEME, bool NT, bool MK, int INTEG, int LAYOUT, int CB, bool FIXED = false, int NTH = CB>
+__global__ void __launch_bounds__(NTH) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out,
                                                       const double *Win, double *Wout, DevFree q, double *W0,
                                                       double alpha, int firstStage, int machDerived,
                                                       double *rmsPartial, int xcds, int reverse) {
@@ -755,14 +758,15 @@ __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState
 	const int e = c0 + t;
 	const bool own = t < nc;
 	// own records: the block's consecutive 64 B records are read as consecutive 16 B chunks,
-	// thread t taking chunks t, t + 256, t + 512, t + 768, so that every wave instruction covers
-	// whole lines; chunk j holds field pair j & 3 (rho u | v w | p E | H M) of cell j >> 2
+	// thread t taking chunks t, t + NTH, ..., so that every wave instruction covers whole lines;
+	// chunk j holds field pair j & 3 (rho u | v w | p E | H M) of cell j >> 2
+	constexpr int kRec = (4 * CB + NTH - 1) / NTH, kCons = (5 * CB / 2 + NTH - 1) / NTH;
 	double2 oc4[4];
 	if constexpr (LAYOUT < 2) {
 		const double2 *blk = reinterpret_cast<const double2 *>(cellPtr(in, c0));
 #pragma unroll
-		for (int i = 0; i < 4; i++) {
-			const int j = i * CB + t;
+		for (int i = 0; i < kRec; i++) {
+			const int j = i * NTH + t;
 			oc4[i] = j < 4 * nc ? blk[j] : make_double2(0, 0);
 		}
 	} else {
@@ -771,11 +775,10 @@ __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState
 		// chunk of the last block may reach one double into the buffer's padding)
 		const double2 *blk = reinterpret_cast<const double2 *>(Win + (size_t) 5 * c0);
 #pragma unroll
-		for (int i = 0; i < 3; i++) {
-			const int j = i * CB + t;
+		for (int i = 0; i < kCons; i++) {
+			const int j = i * NTH + t;
 			oc4[i] = 2 * j < 5 * nc ? blk[j] : make_double2(0, 0);
 		}
-		oc4[3] = make_double2(0, 0);
 	}
 	double w0[5], vol = 0, sx = 0, sy = 0, sz = 0;
 	uint16_t code[6];
@@ -800,7 +803,7 @@ __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState
 			for (int i = 0; i < 6; i++) code[i] = i < p.csrStride ? ldStream<NT>(cl + i) : (uint16_t) 0xFFFF;
 		}
 	}
-	for (int h = t; h < nh; h += CB) {
+	for (int h = t; h < nh; h += NTH) {
 		const int l = CB + h;
 		if constexpr (LAYOUT < 2) {
 			const double2 *hc = reinterpret_cast<const double2 *>(cellPtr(in, p.haloCell[h0 + h]));
@@ -829,8 +832,8 @@ __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState
 	if constexpr (LAYOUT == 2) {
 		// the own W range as loaded, into LDS; turned into primitives after the barrier
 #pragma unroll
-		for (int i = 0; i < 3; i++) {
-			const int j = i * CB + t;
+		for (int i = 0; i < kCons; i++) {
+			const int j = i * NTH + t;
 			if (2 * j < 5 * nc) {
 				lf[2 * j] = oc4[i].x;
 				lf[2 * j + 1] = oc4[i].y;
@@ -843,8 +846,8 @@ __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState
 		double *dA = ls + (k < 2 ? 2 * k : 2 + k) * cap;
 		double *dB = k < 2 ? ls + (2 * k + 1) * cap : lem + (k - 2) * CB;
 #pragma unroll
-		for (int i = 0; i < 4; i++) {
-			const int j = i * CB + t;
+		for (int i = 0; i < kRec; i++) {
+			const int j = i * NTH + t;
 			if (j < 4 * nc) {
 				dA[j >> 2] = oc4[i].x;
 				dB[j >> 2] = oc4[i].y;
@@ -883,14 +886,14 @@ __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState
 	auto lprim = [&](int l) -> Prim {
 		return {ls[l], ls[cap + l], ls[2 * cap + l], ls[3 * cap + l], ls[4 * cap + l], ls[5 * cap + l]};
 	};
-	for (int j = t; j < ns; j += CB) {
+	for (int j = t; j < ns; j += NTH) {
 		const int s = s0 + j;
 		const uint32_t pk = pkN;
 		const double nx = nxN, ny = nyN, nz = nzN, S = SN;
-		if (j + CB < ns) {
-			pkN = ldStream<NT>(p.pack + s + CB);
-			nxN = ldStream<NT>(p.nx + s + CB), nyN = ldStream<NT>(p.ny + s + CB);
-			nzN = ldStream<NT>(p.nz + s + CB), SN = ldStream<NT>(p.S + s + CB);
+		if (j + NTH < ns) {
+			pkN = ldStream<NT>(p.pack + s + NTH);
+			nxN = ldStream<NT>(p.nx + s + NTH), nyN = ldStream<NT>(p.ny + s + NTH);
+			nzN = ldStream<NT>(p.nz + s + NTH), SN = ldStream<NT>(p.S + s + NTH);
 		}
 		const int l0 = (int) (pk & 0x7FFu), l1 = (int) ((pk >> 11) & 0x7FFu), kind = (int) (pk >> 22);
 		const Prim L = lprim(l0);
@@ -1028,7 +1031,7 @@ __global__ void __launch_bounds__(CB) kStageFused(DevMesh m, DevPlan p, DevState
 	}
 	if (rmsPartial) {
 		__syncthreads();
-		double tt = blockSumN<CB>(r2, lds);
+		double tt = blockSumN<NTH>(r2, lds);
 		if (t == 0) rmsPartial[b] = tt;
 	}
 }
@@ -1256,12 +1259,18 @@ void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState 
 	const int rev = t.alternate ? (parity & 1) : 0;
 	const size_t lds = stageFusedLdsBytes(p.cellsPerBlock, p.maxSlots, p.maxHalo, integ == 1 ? 0 : layout);
 	const bool fixed = integ != 1 && stageFusedFixed(p.cellsPerBlock, p.maxSlots, p.maxHalo, layout);
+	// 256-cell blocks whose LDS leaves room for one workgroup per CU (hexahedral meshes with large
+	// halos) run with 512 threads, so that two waves per SIMD share it
+	const bool wide = integ != 1 && p.cellsPerBlock == 256 && 2 * lds > 160 * 1024;
 	const int x = t.xcds;
 #define E3D_LAUNCH_B(S, N, K, B)                                                                               \
 	do {                                                                                                       \
 		if (integ == 1)                                                                                        \
 			hipLaunchKernelGGL((kStageFused<S, N, K, 1, 0, B>), dim3(p.nBlocks), dim3(B), lds, st, m, p, in, out, \
 			                   Win, Wout, q, W0, alpha, 0, machDerived, rmsPartial, x, rev);                      \
+		else if (layout == 2 && wide)                                                                          \
+			hipLaunchKernelGGL((kStageFused<S, N, K, 0, 2, 256, false, 512>), dim3(p.nBlocks), dim3(512), lds, st, m, \
+			                   p, in, out, Win, Wout, q, W0, alpha, firstStage, machDerived, rmsPartial, x, rev); \
 		else if (layout == 2 && fixed)                                                                         \
 			hipLaunchKernelGGL((kStageFused<S, N, K, 0, 2, 256, true>), dim3(p.nBlocks), dim3(256), lds, st, m, p, \
 			                   in, out, Win, Wout, q, W0, alpha, firstStage, machDerived, rmsPartial, x, rev);     \
