@@ -350,12 +350,21 @@ def bench_wing(a) -> dict:
     iters = a.max_iter
     prm = default_params(mach=0.84, aoa_deg=3.0, gamma=1.4, cfl=1.5, sref=1.0, cref=1.0, max_iter=iters,
                          orient_cl=(1, 1), orient_cd=(0, 1), orient_cm=(2, 1))
+    n_faces = m.info()["n_face"]
     with EulerSolver(m, prm) as s:
+        s.set_kernel_timing(True)
         t0 = time.perf_counter()
         r64 = s.run()
         wall64 = time.perf_counter() - t0
+        kern_ms, launches, kname = s.kernel_timing()
         rho64 = s.get_state()["rho"][:n]
-    log(f"fp64: {r64['iterations']} iterations in {wall64:.2f} s, CL {r64['CL']:.9f} CD {r64['CD']:.9f}")
+    bpc = algorithmic_bytes_per_cell_stage(n, n_faces)
+    achieved = bpc * n / (kern_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "kernel": kname, "kernel_ms": kern_ms, "launches": launches,
+            "bytes_per_cell_stage": bpc, "traffic": measured_traffic(kname, n, 0)}
+    log(f"fp64: {r64['iterations']} iterations in {wall64:.2f} s, CL {r64['CL']:.9f} CD {r64['CD']:.9f}; "
+        f"{kname} {kern_ms * 1e3:.1f} us, {roof['frac']:.3f} of HBM peak")
     with Batch(max_cases=1) as b:
         cid = b.add(m, prm, drop=0.0, max_iter=iters)
         t0 = time.perf_counter()
@@ -375,7 +384,7 @@ def bench_wing(a) -> dict:
                        "cells": n},
             "fp64": {"CL": r64["CL"], "CD": r64["CD"], "iterations": r64["iterations"], "rms_first": float(rows[0, 1]),
                      "rms_final": float(rows[-1, 1]), "rms_floor": float(rows[:, 1].min()),
-                     "wall_s": wall64, "cell_stage_per_s": 5 * n * r64["iterations"] / wall64},
+                     "wall_s": wall64, "cell_stage_per_s": 5 * n * r64["iterations"] / wall64, "roofline": roof},
             "mixed": {"CL": res["CL"], "CD": res["CD"], "iterations": len(h32), "rms_first": float(h32[0]),
                       "rms_final": float(h32[-1]), "rms_floor": float(h32.min()),
                       "wall_s": wall32, "cell_stage_per_s": 5 * n * len(res["rms"]) / wall32},

@@ -88,7 +88,7 @@ struct e3d_ctx {
 	// records in st[cur]: the fused RK stage keeps it so, readers of primitives convert first
 	DBuf<double> wst[2];
 	bool wMode = false;
-	DBuf<double> W0, fs, partial, scalars;
+	DBuf<double> W0, fs, partial, sumTmp, scalars; // sumTmp: chunk sums of the RMS partials
 	int partialCount = 0; // block partials the last RMS-forming stage wrote
 	// residual smoothing (RESIDUAL_SMOOTHING= 1): neighbour lists in device numbering, residual
 	// before/during/after the sweeps, per-block largest change, decision and NaN flags
@@ -239,7 +239,7 @@ void runIteration(e3d_ctx *c, double *dsum, bool timed = true) {
 	} else {
 		runStage(c, 0, c->partial.p, timed); // explicit Euler step, or (implicit) residual only
 	}
-	e3d::launchSum(c->partial.p, c->partialCount, dsum, c->stream);
+	e3d::launchSum(c->partial.p, c->partialCount, c->sumTmp.p, dsum, c->stream);
 }
 
 }// namespace
@@ -766,6 +766,7 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		}
 		c->fs.alloc(5 * nF);
 		c->partial.alloc(std::max({1, e3d::stageBlocks(c->nElem), P.nBlocks}));
+		c->sumTmp.alloc(e3d::sumScratch((int) c->partial.n));
 		c->scalars.alloc(16);
 
 		c->dm.nElem = c->nElem;

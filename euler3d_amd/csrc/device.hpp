@@ -140,8 +140,10 @@ void launchWToPrim(const DevMesh &m, const double *W, DevState s, const DevFree 
 constexpr int kIterScratchMax = 65536;
 hipError_t iterScratchInit(int device);
 double *iterScratch(int device);
-// deterministic sum of n doubles into out[0] (single workgroup, fixed order)
-void launchSum(const double *in, int n, double *out, hipStream_t st);
+// deterministic sum of n doubles into out[0] (fixed order, over several workgroups); tmp holds
+// sumScratch(n) doubles
+int sumScratch(int n);
+void launchSum(const double *in, int n, double *tmp, double *out, hipStream_t st);
 // interior Mach number from the primitives (FlowField.cpp:157), for state read-back
 void launchMach(const DevMesh &m, DevState s, const DevFree &q, hipStream_t st);
 // wall-face force/moment: out[6] = (Fx,Fy,Fz, CPx,CPy,CPz) summed in a fixed order

@@ -21,6 +21,7 @@
 //   kBc, kFlux, kStage (+ kGather, kTimestep)  one pass per step through HBM, used by the
 //                per-face parity entry points (update_bc, face_flux, residual, timestep).
 #include "device.hpp"
+#include "reduce.hpp"
 
 #include <algorithm>
 #include <mutex>
@@ -448,28 +449,11 @@ __global__ void __launch_bounds__(kBlock) kTimestep(DevMesh m, DevState s, DevFr
 	dt[e] = localDt(m, q, P.rho, P.u, P.v, P.w, P.p, e);
 }
 
-// deterministic block reduction of one double per thread (fixed tree)
-// the same fixed tree for a workgroup of N threads
+// deterministic reductions of one double per thread: wavefront butterflies, then the wave totals
+// in order (reduce.hpp)
 template<int N>
-__device__ __forceinline__ double blockSumN(double x, double *lds) {
-	lds[threadIdx.x] = x;
-	__syncthreads();
-	for (int off = N / 2; off > 0; off >>= 1) {
-		if ((int) threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
-		__syncthreads();
-	}
-	return lds[0];
-}
-
-__device__ __forceinline__ double blockSum(double x, double *lds) {
-	lds[threadIdx.x] = x;
-	__syncthreads();
-	for (int off = kBlock / 2; off > 0; off >>= 1) {
-		if ((int) threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
-		__syncthreads();
-	}
-	return lds[0];
-}
+__device__ __forceinline__ double blockSumN(double x, double *lds) { return groupSum<N>(x, lds); }
+__device__ __forceinline__ double blockSum(double x, double *lds) { return groupSum<kBlock>(x, lds); }
 
 // RK stage update of one cell from its residual r: EulerSolver.cpp:431-449 / 459-470 and
 // FlowField.cpp:140-158. (rho, u, v, w, p, E) is the cell's state before the stage; writes `out`
@@ -1082,20 +1066,25 @@ __global__ void __launch_bounds__(kBlock) kWToPrim(DevMesh m, const double *W, D
 	c[3] = make_double2(H, machOf(r, u, v, w, pr, q.gamma));
 }
 
-// deterministic sum of n doubles: each of kSumThreads threads adds the strided elements
-// i = t, t + kSumThreads, ... in index order (independent loads, coalesced), then a fixed tree
-constexpr int kSumThreads = 1024;
-__global__ void __launch_bounds__(kSumThreads) kSum(const double *in, int n, double *out) {
-	__shared__ double lds[kSumThreads];
+// deterministic sum of n doubles in two launches over several workgroups: kSumChunks reduces
+// chunk g = [1024 g, 1024 g + 1024) (each thread adds its four elements t, t + 256, t + 512,
+// t + 768 in that order, then groupSum) into tmp[g]; kSumTotal adds the chunk sums the same way.
+constexpr int kSumThreads = 256, kSumChunk = 4 * kSumThreads;
+__global__ void __launch_bounds__(kSumThreads) kSumChunks(const double *in, int n, double *tmp) {
+	__shared__ double lds[kSumThreads / 64];
+	const int base = blockIdx.x * kSumChunk + threadIdx.x;
+	double v[4];
+#pragma unroll
+	for (int i = 0; i < 4; i++) v[i] = base + i * kSumThreads < n ? in[base + i * kSumThreads] : 0.0;
+	const double s = groupSum<kSumThreads>(((v[0] + v[1]) + v[2]) + v[3], lds);
+	if (threadIdx.x == 0) tmp[blockIdx.x] = s;
+}
+__global__ void __launch_bounds__(kSumThreads) kSumTotal(const double *tmp, int chunks, double *out) {
+	__shared__ double lds[kSumThreads / 64];
 	double acc = 0.0;
-	for (int i = threadIdx.x; i < n; i += kSumThreads) acc += in[i];
-	lds[threadIdx.x] = acc;
-	__syncthreads();
-	for (int off = kSumThreads / 2; off > 0; off >>= 1) {
-		if ((int) threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
-		__syncthreads();
-	}
-	if (threadIdx.x == 0) out[0] = lds[0];
+	for (int i = threadIdx.x; i < chunks; i += kSumThreads) acc += tmp[i];
+	const double s = groupSum<kSumThreads>(acc, lds);
+	if (threadIdx.x == 0) out[0] = s;
 }
 
 __global__ void __launch_bounds__(kBlock) kMach(DevMesh m, DevState s, DevFree q) {
@@ -1331,8 +1320,12 @@ double *iterScratch(int device) {
 	return gScratch[device];
 }
 
-void launchSum(const double *in, int n, double *out, hipStream_t st) {
-	hipLaunchKernelGGL(kSum, dim3(1), dim3(kSumThreads), 0, st, in, n, out);
+int sumScratch(int n) { return std::max(1, (n + kSumChunk - 1) / kSumChunk); }
+
+void launchSum(const double *in, int n, double *tmp, double *out, hipStream_t st) {
+	const int chunks = sumScratch(n);
+	hipLaunchKernelGGL(kSumChunks, dim3(chunks), dim3(kSumThreads), 0, st, in, n, tmp);
+	hipLaunchKernelGGL(kSumTotal, dim3(1), dim3(kSumThreads), 0, st, tmp, chunks, out);
 }
 
 void launchMach(const DevMesh &m, DevState s, const DevFree &q, hipStream_t st) {

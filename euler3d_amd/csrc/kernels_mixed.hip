@@ -15,6 +15,7 @@
 // a freestream state still gives an exact zero. The ghost states, the time step and the FP64
 // update keep separately rounded products.
 #include "mixed.hpp"
+#include "reduce.hpp"
 
 namespace e3d {
 
@@ -256,14 +257,9 @@ __device__ __forceinline__ int logicalBlock(int w, int n, int xcds, bool reverse
 	return x * q + min(x, r) + (reverse ? len - 1 - k : k);
 }
 
+// deterministic workgroup sum (reduce.hpp); the trailing barrier frees lds for the caller
 __device__ __forceinline__ double blockSumD(double x, double *lds) {
-	lds[threadIdx.x] = x;
-	__syncthreads();
-	for (int off = kB / 2; off > 0; off >>= 1) {
-		if ((int) threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
-		__syncthreads();
-	}
-	const double s = lds[0];
+	const double s = groupSum<kB>(x, lds);
 	__syncthreads();
 	return s;
 }
@@ -493,21 +489,16 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 // residual history, convergence and buffer parity
 constexpr int kEndThreads = 1024;
 __global__ void __launch_bounds__(kEndThreads) kMixIterEnd(const MixCase *cases, MixStatus *status) {
-	__shared__ double red[kEndThreads];
+	__shared__ double red[kEndThreads / 64];
 	const int slot = blockIdx.x;
 	MixStatus &st = status[slot];
 	if (!st.active || st.done) return;
 	const MixCase &C = cases[slot];
 	double acc = 0.0;
 	for (int i = threadIdx.x; i < C.nBlocks; i += kEndThreads) acc += C.partial[i];
-	red[threadIdx.x] = acc;
-	__syncthreads();
-	for (int off = kEndThreads / 2; off > 0; off >>= 1) {
-		if ((int) threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-		__syncthreads();
-	}
+	const double sum = groupSum<kEndThreads>(acc, red);
 	if (threadIdx.x == 0) {
-		const double rms = sqrt(red[0] / C.nElem);
+		const double rms = sqrt(sum / C.nElem);
 		const int it = st.iter;
 		if (it < C.histCap) C.hist[it] = rms;
 		if (it == 0) st.rms0 = rms;
@@ -564,7 +555,7 @@ __global__ void __launch_bounds__(256) kMixAero(int nWall, const int *wallCell, 
                                                 const double *Nz, const double *Cx, const double *Cy, const double *Cz,
                                                 const MixCase *dc, double cpFactor, double pInf, double cref,
                                                 double sref, double *out6) {
-	__shared__ double red[256];
+	__shared__ double red[256 / 64];
 	const MixCase &C = *dc;
 	const size_t n = (size_t) C.nElem;
 	double acc[6] = {0, 0, 0, 0, 0, 0};
@@ -581,13 +572,8 @@ __global__ void __launch_bounds__(256) kMixAero(int nWall, const int *wallCell, 
 		acc[3] += cx * Cx[i], acc[4] += cy * Cy[i], acc[5] += cz * Cz[i];
 	}
 	for (int k = 0; k < 6; k++) {
-		red[threadIdx.x] = acc[k];
-		__syncthreads();
-		for (int off = 128; off > 0; off >>= 1) {
-			if ((int) threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-			__syncthreads();
-		}
-		if (threadIdx.x == 0) out6[k] = red[0];
+		const double t = groupSum<256>(acc[k], red);
+		if (threadIdx.x == 0) out6[k] = t;
 		__syncthreads();
 	}
 }
