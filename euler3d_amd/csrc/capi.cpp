@@ -110,6 +110,14 @@ struct e3d_ctx {
 	DBuf<double> aeroOut;
 	DBuf<double> wallCp;  // cp of every wall face at the last force update (AeroCoefficients::_cp)
 	bool wallCpSet = false;
+	// partitioned input with the interface exchange (E3D_EXCHANGE=halo): interface faces read their
+	// ghosts, which launchHaloExchange fills from the send cells before every stage and in
+	// e3d_update_bc; exchanged = the ghosts hold exchanged values (after e3d_update_bc) rather
+	// than the copy of their own cell that an update leaves (FlowField.cpp:129-138)
+	bool halo = false, exchanged = false;
+	int nX = 0;
+	DBuf<int> xSrc, xDst, xSlot;
+	DBuf<double> xBuf;
 
 	e3d::DevState state(int which) { return {st[which].p}; }
 	e3d::DevState state() { return state(cur); }
@@ -123,6 +131,14 @@ struct e3d_ctx {
 	}
 	void sync() { hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
 	void check() { hipCheck(hipGetLastError(), "kernel launch"); }
+	// MPIHandler::updateFlowField on one device: send cells -> message buffer -> ghost records of
+	// the state the next stage reads
+	void exchange() {
+		if (!halo) return;
+		e3d::launchHaloExchange(nX, xSrc.p, xDst.p, xSlot.p, state(), wMode ? wst[cur].p : nullptr, fr.gamma, xBuf.p,
+		                        stream);
+		check();
+	}
 };
 
 namespace {
@@ -167,6 +183,7 @@ void runStage(e3d_ctx *c, int stage, double *rmsPartial, bool timed) {
 	}
 	const int ti = c->prm.time_integration;
 	const bool smooth = c->prm.residual_smoothing != 0;
+	c->exchange(); // updateBC's interface exchange (EulerSolver.cpp:248-250), before the stage reads
 	if (c->fused && !smooth && ti != 1) {
 		// one fused launch: ghosts, fluxes, gather, dt, update (RK stage or explicit Euler step);
 		// RK stages leave the interior state as W, the explicit Euler step as primitives
@@ -591,6 +608,11 @@ int e3d_mesh_get_i32(const e3d_mesh *m, const char *name, int *out, long long ca
 	else if (s == "mpiGhost") v = &sm.mpiGhost;
 	else if (s == "mpiCell") v = &sm.mpiCell;
 	else if (s == "partStart") v = &sm.partStart;
+	else if (s == "haloStart") v = &sm.haloStart;
+	else if (s == "haloFrom") v = &sm.haloFrom;
+	else if (s == "haloTo") v = &sm.haloTo;
+	else if (s == "haloSrc") v = &sm.haloSrc;
+	else if (s == "haloDst") v = &sm.haloDst;
 	else if (s == "elemVtk") v = &m->file.elemVtk;
 	else if (s == "elemStart") v = &m->file.elemStart;
 	else if (s == "elemNodes") v = &m->file.elemNodes;
@@ -668,8 +690,19 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		// more: the 1.18M-hex wing ran 2.48 s against 2.27 s per 2,000 iterations.)
 		int cells = 0;
 		if (const char *v = std::getenv("E3D_BLOCK_CELLS")) cells = std::atoi(v) == 128 ? 128 : 256;
-		e3d::HostPlan P = e3d::buildPlan(sm, order, cells ? cells : e3d::kCellsPerBlock);
-		if (!cells && e3d::stageFusedWavesPerCU(256, P.maxSlots, P.maxHalo, p->scheme) == 0) P = e3d::buildPlan(sm, order, 128);
+		// partitioned input: interface faces read the other partition's cell (default), or with
+		// E3D_EXCHANGE=halo their ghosts, filled by the exchange of MPIHandler::updateFlowField
+		if (const char *v = std::getenv("E3D_EXCHANGE")) c->halo = std::string(v) == "halo" && !sm.haloSrc.empty();
+		e3d::HostPlan P = e3d::buildPlan(sm, order, cells ? cells : e3d::kCellsPerBlock, c->halo);
+		if (!cells && e3d::stageFusedWavesPerCU(256, P.maxSlots, P.maxHalo, p->scheme) == 0)
+			P = e3d::buildPlan(sm, order, 128, c->halo);
+		if (c->halo) {
+			c->nX = (int) P.xSrc.size();
+			c->xSrc.alloc(c->nX), c->xSrc.upload(P.xSrc.data(), c->nX, s);
+			c->xDst.alloc(c->nX), c->xDst.upload(P.xDst.data(), c->nX, s);
+			c->xSlot.alloc(2 * (size_t) c->nX), c->xSlot.upload(P.xSlot.data(), 2 * (size_t) c->nX, s);
+			c->xBuf.alloc(std::max(1, P.xBufLen));
+		}
 		if (e3d::stageFusedWavesPerCU(P.cellsPerBlock, P.maxSlots, P.maxHalo, p->scheme) == 0)
 			return fail(E3D_ERR_UNSUPPORTED, "a block of the fused stage needs more LDS than a CU has");
 		c->new2old = P.new2old;
@@ -803,6 +836,7 @@ int e3d_set_state(e3d_ctx *c, const double *const fields[8]) {
 		c->sync();
 		c->machDerived = 0;
 		c->wMode = false;
+		c->exchanged = false;
 		return E3D_OK;
 	});
 }
@@ -824,7 +858,8 @@ int e3d_get_state(e3d_ctx *c, double *const fields[8]) {
 			if (!fields[k]) continue;
 			toReference(c, h, k, fields[k]);
 			// an interface ghost holds the copy of its own cell that the last update left in it
-			// (FlowField.cpp:129-138, 159-169)
+			// (FlowField.cpp:129-138, 159-169), unless an exchange has filled it since
+			if (c->halo && c->exchanged) continue;
 			for (size_t i = 0; i < c->mpiGhost.size(); i++) fields[k][c->mpiGhost[i]] = fields[k][c->mpiCell[i]];
 		}
 		return E3D_OK;
@@ -838,6 +873,8 @@ int e3d_update_bc(e3d_ctx *c) {
 		c->ensurePrims();
 		e3d::launchBc(c->dm, c->state(), c->fr, c->machDerived, c->stream);
 		c->check();
+		c->exchange();
+		c->exchanged = c->halo;
 		c->sync();
 		return E3D_OK;
 	});
@@ -917,6 +954,7 @@ int e3d_iterate(e3d_ctx *c, int n, double *rms_sum) {
 	if (!c || n < 0) return fail(E3D_ERR_ARG, "bad argument");
 	return guarded([&] {
 		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		c->exchanged = false;
 		DBuf<double> sums;
 		sums.alloc(std::max(1, n));
 		hipCheck(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
@@ -941,6 +979,7 @@ int e3d_iterate_noalloc(e3d_ctx *c, int n, double *rms_sum) {
 		                             std::to_string(e3d::kIterScratchMax) + " its static scratch holds");
 	return guarded([&] {
 		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		c->exchanged = false;
 		double *sums = e3d::iterScratch(c->device);
 		if (!sums) return fail(E3D_ERR_DEVICE, "static scratch not set up on this device");
 		hipCheck(hipEventRecord(c->ev0, c->stream), "hipEventRecord");
@@ -1023,6 +1062,7 @@ int e3d_run(e3d_ctx *c, double *rows, int max_rows, int *n_rows, int *iters, dou
 	if (!c) return fail(E3D_ERR_ARG, "null context");
 	return guarded([&] {
 		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		c->exchanged = false;
 		const e3d_params &q = c->prm;
 		int nb = 0, counter = 0, nrows = 0;
 		double CL = 0, CD = 0, CM = 0, cf[3] = {0, 0, 0}, cm[3] = {0, 0, 0};
@@ -1125,6 +1165,7 @@ int e3d_checkpoint_read(e3d_ctx *c, const char *path) {
 	if (!c || !path) return fail(E3D_ERR_ARG, "null argument");
 	return guarded([&] {
 		hipCheck(hipSetDevice(c->device), "hipSetDevice");
+		c->exchanged = false;
 		std::FILE *f = std::fopen(path, "rb");
 		if (!f) return fail(E3D_ERR_IO, std::string("cannot open ") + path);
 		CkptHeader h{};

@@ -129,6 +129,12 @@ void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState 
 // bitwise comparison of the short square root, reciprocal and quotient with the library's on n
 // pseudo-random inputs; bad[0..2] (device, zeroed by the caller) count the mismatches
 void launchMathCheck(unsigned long long seed, long long n, unsigned long long *bad, hipStream_t st);
+// the interface exchange of a partitioned mesh on one device: pack (rho, u, v, w, E) of the n send
+// cells src[] into the message buffer (from W when W is not null, else from the primitive
+// records of s), then unpack into the ghost records dst[] of s with p, H and M re-formed; slot
+// holds 2 ints per entry (buffer offset, message length)
+void launchHaloExchange(int n, const int *src, const int *dst, const int *slot, DevState s, const double *W,
+                        double gamma, double *buf, hipStream_t st);
 // dst[i] = src[i] for `chunks` 16 B chunks (grid-stride; both 16 B aligned)
 void launchCopy16(const void *src, void *dst, long long chunks, hipStream_t st);
 // interior primitive records of s from the conservative states W (the stage's expressions), with

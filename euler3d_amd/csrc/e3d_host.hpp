@@ -87,6 +87,13 @@ struct SolverMesh {
 	// cells [partStart[r], partStart[r + 1]) belong to partition r of a composite mesh (empty for
 	// a mesh read on its own)
 	std::vector<int> partStart;
+	// the interface exchange of a composite mesh (MPIHandler::updateFlowField), one message per
+	// (sending, receiving) partition pair: entries [haloStart[k], haloStart[k + 1]) of message k
+	// carry the state of cell haloSrc[i] into ghost haloDst[i] (composite ids), in the order the
+	// reference's sorted requester and sender lists pair them; haloFrom[k] / haloTo[k] are the
+	// sending and receiving partitions. A ghost that no exchange reaches gets its own cell
+	// (a message from its partition to itself, last).
+	std::vector<int> haloStart, haloFrom, haloTo, haloSrc, haloDst;
 
 	// metrics (FP64), non-dimensionalised by CREF
 	std::vector<double> faceNormal;     // 3 * nFace, area-weighted, oriented elem0 -> elem1

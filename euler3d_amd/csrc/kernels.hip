@@ -897,6 +897,10 @@ __global__ void __launch_bounds__(NTH) kStageFused(DevMesh m, DevPlan p, DevStat
 		if (kind == 0) {
 			if constexpr (SCHEME == 1 && LAYOUT == 2) ausmFluxA(L, lprim(l1), lcs[l0], lcs[l1], nx, ny, nz, q.gamma, F);
 			else schemeFlux<SCHEME>(L, lprim(l1), nx, ny, nz, q.gamma, F);
+		} else if (kind == 4) {
+			// partition interface read from its ghost, which the interface exchange filled
+			// (EulerSolver.cpp:343-346: the scheme's flux with the ghost, to the own cell only)
+			schemeFlux<SCHEME>(L, loadPrim(in, p.e1[s]), nx, ny, nz, q.gamma, F);
 		} else {
 			const int a = c0 + l0;    // the interior cell of a boundary face is an own cell
 			const int c = p.e1[s];    // its ghost
@@ -1133,6 +1137,53 @@ __global__ void __launch_bounds__(kBlock) kAero(int nWall, const int *wallCell, 
 	}
 }
 
+// Interface exchange between partitions (MPIHandler::updateFlowField, MPIHandler.cpp:84-267).
+// Pack: entry i puts (rho, u, v, w, E) of interior cell src[i] into its message at buf[slot.x +
+// k * slot.y], k = rho, u, v, w, E (each message holds its rho's, then its u's, v's, w's and E's,
+// as the reference sends them). The state is read from the conservative states W when W is given
+// (primitives formed as the update forms them), else from the primitive records.
+__global__ void __launch_bounds__(kBlock) kHaloPack(int n, const int *src, const int2 *slot, DevState s,
+                                                    const double *W, double gamma, double *buf) {
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n) return;
+	const int c = src[i];
+	double v[5];
+	if (W) {
+		const double *w5 = W + (size_t) 5 * c;
+		double pr, H;
+		v[0] = w5[0];
+		primOf<true>(w5[0], w5[1], w5[2], w5[3], w5[4], gamma, v[1], v[2], v[3], pr, v[4], H);
+	} else {
+		const double *r = cellPtr(s, c);
+		v[0] = r[S_RHO], v[1] = r[S_U], v[2] = r[S_V], v[3] = r[S_W], v[4] = r[S_E];
+	}
+	const int2 sl = slot[i];
+#pragma unroll
+	for (int k = 0; k < 5; k++) buf[sl.x + k * sl.y] = v[k];
+}
+
+// Unpack: ghost dst[i] takes (rho, u, v, w, E) from its message; p, H and M are formed as the
+// reference's receiving rank forms them (MPIHandler.cpp:146-165): p = (gamma - 1) rho (E -
+// (u^2 + v^2 + w^2) / 2), H = E + p / rho, M = |u| / sqrt(gamma (p / rho)). p and H are the update's
+// own expressions, so a ghost carries the bits of the cell it copies.
+__global__ void __launch_bounds__(kBlock) kHaloUnpack(int n, const int *dst, const int2 *slot, const double *buf,
+                                                      double gamma, DevState s) {
+	const int i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n) return;
+	const int2 sl = slot[i];
+	const double r = buf[sl.x], u = buf[sl.x + sl.y], v = buf[sl.x + 2 * sl.y], w = buf[sl.x + 3 * sl.y];
+	const double E = buf[sl.x + 4 * sl.y];
+	const double p = (gamma - 1) * r * (E - (u * u + v * v + w * w) / 2.0);
+	const double H = E + p / r;
+	const double a = sqrt(gamma * (p / r));
+	const double M = sqrt(u * u + v * v + w * w) / a;
+	double2 *c = reinterpret_cast<double2 *>(cellPtr(s, dst[i]));
+	c[0] = make_double2(r, u);
+	c[1] = make_double2(v, w);
+	c[2] = make_double2(p, E);
+	c[3] = make_double2(H, M);
+}
+
 // plain copy of n 16 B chunks, each read once and written once (a kernel of known traffic for
 // scaling the memory counters)
 __global__ void __launch_bounds__(kBlock) kCopy16(const double2 *src, double2 *dst, long long n) {
@@ -1174,6 +1225,14 @@ int stageFusedWavesPerCU(int cellsPerBlock, int maxSlots, int maxHalo, int schem
 		                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kStageFused<0, true, true, 0, 2, 256>, 256, lds);
 	if (e != hipSuccess || n <= 0) n = 1;
 	return n * cellsPerBlock / 64;
+}
+
+void launchHaloExchange(int n, const int *src, const int *dst, const int *slot, DevState s, const double *W,
+                        double gamma, double *buf, hipStream_t st) {
+	if (n <= 0) return;
+	const int2 *sl = reinterpret_cast<const int2 *>(slot);
+	hipLaunchKernelGGL(kHaloPack, dim3(blocksFor(n)), dim3(kBlock), 0, st, n, src, sl, s, W, gamma, buf);
+	hipLaunchKernelGGL(kHaloUnpack, dim3(blocksFor(n)), dim3(kBlock), 0, st, n, dst, sl, buf, gamma, s);
 }
 
 void launchCopy16(const void *src, void *dst, long long chunks, hipStream_t st) {

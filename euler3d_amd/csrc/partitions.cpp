@@ -174,17 +174,36 @@ SolverMesh mergePartitions(const std::vector<MeshFile> &files, const std::vector
 			const auto &theirs = req[s][jm];
 			if (mine.size() != theirs.size())
 				throw std::runtime_error("partitions " + std::to_string(r) + " and " + std::to_string(s) + " disagree on their interface");
+			if (c.haloStart.empty()) c.haloStart.push_back(0);
+			c.haloFrom.push_back(s), c.haloTo.push_back(r);
 			for (size_t i = 0; i < mine.size(); i++) {
 				const int g = mine[i].ghost;               // local ghost id in r
 				const int f = parts[r].nFaceInt + (g - parts[r].nElem);
 				c.face2element[2 * (size_t) faceG(r, f) + 1] = ranges[s].elem0 + theirs[i].self;
+				c.haloSrc.push_back(ranges[s].elem0 + theirs[i].self);
+				c.haloDst.push_back(cellG(r, g));
 			}
+			c.haloStart.push_back((int) c.haloSrc.size());
 		}
 	}
-	// a ghost that no exchange reaches keeps the copy of its own cell
+	// a ghost that no exchange reaches keeps the copy of its own cell: one message per partition,
+	// from the partition to itself
+	std::vector<std::vector<int>> selfSrc(N), selfDst(N);
 	for (size_t i = 0; i < c.mpiGhost.size(); i++) {
 		const int f = c.nFaceInt + (c.mpiGhost[i] - nE);
-		if (c.face2element[2 * (size_t) f + 1] >= nE) c.face2element[2 * (size_t) f + 1] = c.mpiCell[i];
+		if (c.face2element[2 * (size_t) f + 1] < nE) continue;
+		c.face2element[2 * (size_t) f + 1] = c.mpiCell[i];
+		const int r = (int) (std::upper_bound(c.partStart.begin(), c.partStart.end(), c.mpiCell[i]) - c.partStart.begin()) - 1;
+		selfSrc[r].push_back(c.mpiCell[i]);
+		selfDst[r].push_back(c.mpiGhost[i]);
+	}
+	for (int r = 0; r < N; r++) {
+		if (selfSrc[r].empty()) continue;
+		if (c.haloStart.empty()) c.haloStart.push_back(0);
+		c.haloFrom.push_back(r), c.haloTo.push_back(r);
+		c.haloSrc.insert(c.haloSrc.end(), selfSrc[r].begin(), selfSrc[r].end());
+		c.haloDst.insert(c.haloDst.end(), selfDst[r].begin(), selfDst[r].end());
+		c.haloStart.push_back((int) c.haloSrc.size());
 	}
 
 	// the merged file (node ids offset per partition)

@@ -192,19 +192,21 @@ std::vector<int> bisectOrder(const SolverMesh &sm, int cellsPerBlock) {
 	return ids;
 }
 
-int8_t slotKind(const SolverMesh &sm, int f) {
+int8_t slotKind(const SolverMesh &sm, int f, bool exchange) {
 	if (f < sm.nFaceInt) return 0;
 	switch (sm.ghostKind[f - sm.nFaceInt]) {
 		case BND_WALL: return 1;
 		case BND_SYM: return 2;
-		case BND_MPI: return 0; // the other partition's cell is elem1: a one-sided interior face
+		// the other partition's cell is elem1 (a one-sided interior face), or with the exchange
+		// the ghost it fills
+		case BND_MPI: return exchange ? 4 : 0;
 		default: return 3;
 	}
 }
 
 }// namespace
 
-HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock) {
+HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock, bool exchange) {
 	HostPlan P;
 	const int nE = sm.nElem, nF = sm.nFace;
 	P.nElem = nE;
@@ -229,6 +231,7 @@ HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock) {
 	for (int f = 0; f < nF; f++) {
 		P.fe0[f] = dev(sm.face2element[2 * (size_t) f]);
 		P.fe1[f] = dev(sm.face2element[2 * (size_t) f + 1]);
+		if (exchange && f >= sm.nFaceInt && sm.ghostKind[f - sm.nFaceInt] == BND_MPI) P.fe1[f] = nE + (f - sm.nFaceInt);
 		P.fnx[f] = sm.faceUnitNormal[3 * (size_t) f];
 		P.fny[f] = sm.faceUnitNormal[3 * (size_t) f + 1];
 		P.fnz[f] = sm.faceUnitNormal[3 * (size_t) f + 2];
@@ -300,7 +303,7 @@ HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock) {
 					slotOf[f] = slot;
 					P.se0.push_back(P.fe0[f]);
 					P.se1.push_back(P.fe1[f]);
-					P.skind.push_back(slotKind(sm, f));
+					P.skind.push_back(slotKind(sm, f, exchange));
 					P.snx.push_back(P.fnx[f]);
 					P.sny.push_back(P.fny[f]);
 					P.snz.push_back(P.fnz[f]);
@@ -324,9 +327,9 @@ HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock) {
 		std::vector<int> newIdx;
 		for (int blk = 0; blk < P.nBlocks; blk++) {
 			const int s0 = P.blockSlot[blk], s1 = P.blockSlot[blk + 1], ns = s1 - s0;
-			int count[5] = {0, 0, 0, 0, 0};
+			int count[6] = {0, 0, 0, 0, 0, 0};
 			for (int s = s0; s < s1; s++) count[P.skind[s] + 1]++;
-			for (int k = 1; k < 5; k++) count[k] += count[k - 1];
+			for (int k = 1; k < 6; k++) count[k] += count[k - 1];
 			newIdx.assign(ns, 0);
 			for (int j = 0; j < ns; j++) newIdx[j] = count[P.skind[s0 + j]]++;
 			auto permute = [&](auto &v) {
@@ -369,6 +372,21 @@ HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock) {
 		P.haloCell.insert(P.haloCell.end(), halo.begin(), halo.end());
 		P.blockHalo[blk + 1] = (int) P.haloCell.size();
 		P.maxHalo = std::max(P.maxHalo, (int) halo.size());
+	}
+
+	// the interface exchange: messages keep their entries contiguous in the buffer, as
+	// MPIHandler::updateFlowField sends them (rho's, then u's, v's, w's, E's)
+	if (exchange) {
+		for (size_t k = 0; k + 1 < sm.haloStart.size(); k++) {
+			const int a = sm.haloStart[k], b = sm.haloStart[k + 1], len = b - a;
+			for (int i = a; i < b; i++) {
+				P.xSrc.push_back(dev(sm.haloSrc[i]));
+				P.xDst.push_back(sm.haloDst[i]);
+				P.xSlot.push_back(5 * a + (i - a));
+				P.xSlot.push_back(len);
+			}
+		}
+		P.xBufLen = 5 * (int) P.xSrc.size();
 	}
 
 	// wall list for the force integration, in the reference's order (Mesh.hpp:594-627)

@@ -29,8 +29,8 @@ struct HostPlan {
 	std::vector<int> blockSlot;
 	std::vector<int> se0, se1;
 	std::vector<int8_t> skind;
-	// block-local view of the slots: bits 0-10 local index of e0, 11-21 local index of e1, 22-23
-	// kind. Local index: own cell i -> i - c0; halo cell -> cellsPerBlock + its rank in the
+	// block-local view of the slots: bits 0-10 local index of e0, 11-21 local index of e1, 22-24
+	// kind (0 interior, 1 wall, 2 symmetry, 3 farfield, 4 partition interface read from its ghost). Local index: own cell i -> i - c0; halo cell -> cellsPerBlock + its rank in the
 	// block's halo list; e1 of a boundary slot is a ghost and has no local index (0x7FF).
 	std::vector<uint32_t> spack;
 	std::vector<int> blockHalo;  // nBlocks + 1
@@ -39,11 +39,19 @@ struct HostPlan {
 	std::vector<double> snx, sny, snz, sS;
 	std::vector<uint16_t> csr;
 
+	// interface exchange of a composite mesh when its faces read ghost slots (buildPlan exchange =
+	// true): entry i carries interior cell xSrc[i] (device numbering) into ghost xDst[i]; its values
+	// sit in the message buffer at xSlot[2 i] + k * xSlot[2 i + 1] for k = rho, u, v, w, E
+	std::vector<int> xSrc, xDst, xSlot;
+	int xBufLen = 0;
+
 	// wall list (reference order) for the force integration
 	std::vector<int> wallCell;
 	std::vector<double> wallN[3], wallC[3];
 };
 
-HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock);
+// exchange: the faces of partition interfaces read the ghost that the interface exchange fills
+// (slot kind 4) instead of the other partition's cell, and the plan carries that exchange
+HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock, bool exchange = false);
 
 }// namespace e3d

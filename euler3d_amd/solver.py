@@ -24,7 +24,7 @@ STATE_FIELDS = ("rho", "u", "v", "w", "p", "E", "H", "M")
 I32_ARRAYS = ("face2element", "face2nodeStart", "face2node", "element2faceStart", "element2face",
               "element2element", "wallGhost", "wallCell", "wallFace", "farGhost", "farCell", "farFace",
               "symGhost", "symCell", "symFace", "ghostKind", "elemVtk", "elemStart", "elemNodes", "mpiGhost",
-              "mpiCell", "partStart")
+              "mpiCell", "partStart", "haloStart", "haloFrom", "haloTo", "haloSrc", "haloDst")
 F64_ARRAYS = ("faceNormals", "faceNormalsUnit", "faceSurfaces", "faceCenters", "cellVolumes",
               "cellCentroids", "xyz")
 

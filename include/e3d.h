@@ -159,7 +159,11 @@ int e3d_mesh_get_info(const e3d_mesh *m, e3d_mesh_info *out);
 /* Copies a named array out (capacity in elements; *n receives the full length). Names:
  * int32: face2element, face2nodeStart, face2node, element2faceStart, element2face,
  *        element2element, wallGhost, wallCell, wallFace, farGhost, farCell, farFace, symGhost,
- *        symCell, symFace, ghostKind, elemVtk, elemStart, elemNodes
+ *        symCell, symFace, ghostKind, elemVtk, elemStart, elemNodes, mpiGhost, mpiCell, partStart,
+ *        and of several partitions read together the interface exchange (MPIHandler::
+ *        updateFlowField, one message per sending/receiving partition pair): haloStart (message
+ *        offsets), haloFrom, haloTo (its partitions), haloSrc (sent cells), haloDst (receiving
+ *        ghosts)
  * f64:   faceNormals, faceNormalsUnit, faceSurfaces, faceCenters, cellVolumes, cellCentroids, xyz */
 int e3d_mesh_get_i32(const e3d_mesh *m, const char *name, int *out, long long capacity, long long *n);
 int e3d_mesh_get_f64(const e3d_mesh *m, const char *name, double *out, long long capacity, long long *n);

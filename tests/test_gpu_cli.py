@@ -88,14 +88,16 @@ def test_cli_matches_reference_outputs(case, tmp_path):
     assert len(lines) == 4 + 3 * nn + 7 * n
 
 
+@pytest.mark.parametrize("exchange", ["direct", "halo"])
 @pytest.mark.parametrize("case", ["block8_p4", "naca_small_p3", "bump_small_p4_smooth", "wing_small_p2_euler_smooth"])
-def test_cli_partitioned_matches_reference_outputs(case, tmp_path):
+def test_cli_partitioned_matches_reference_outputs(case, exchange, tmp_path, monkeypatch):
     """PARTITION_FILES= N: every partition resident on the one device, outputs as the reference's
     `mpiexec -np N` run writes them (residuals.dat, summary.txt with Ncores = N, per-partition
-    .sol and cp_part_<r>.dat)."""
+    .sol and cp_part_<r>.dat), with interfaces read directly or through the exchange."""
     import gzip
 
-    print(f"[gpu] E3D_Solver partitioned case={case}", flush=True)
+    print(f"[gpu] E3D_Solver partitioned case={case} exchange={exchange}", flush=True)
+    monkeypatch.setenv("E3D_EXCHANGE", exchange)
     g = load_golden(case)
     pd = integrator_params(g)
     p = g["params"]

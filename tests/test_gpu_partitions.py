@@ -3,9 +3,14 @@ as one resident mesh, against the reference's ranks (tests/golden/<case>.npz, r<
 
 Bar, as for one partition: bit-exact FP64 for ghost states, per-face fluxes (the rank's own face
 order), residuals, time steps and states after iterations, for every partition; the RMS and force
-sums at 1e-12 relative (fixed-tree reductions against the reference's sequential sums). Interface
-ghosts are compared after an update (they hold their own cell's copy), not after updateBC (where
-the reference holds the other partition's cell, which the device reads directly).
+sums at 1e-12 relative (fixed-tree reductions against the reference's sequential sums).
+
+Both ways of reading a partition interface run: "direct" (the face reads the other partition's cell)
+and "halo" (E3D_EXCHANGE=halo: per-pair messages packed from the send cells into a device buffer and
+unpacked into the interface ghosts before every stage, MPIHandler::updateFlowField). With the
+exchange the interface ghosts are also compared after updateBC, where the reference holds the
+values the other rank sent; directly, they are compared after an update only (their own cell's
+copy).
 """
 from __future__ import annotations
 
@@ -38,15 +43,17 @@ def seeds_of(g):
     return sorted({int(k.split("_")[1][4:]) for k in g if k.startswith("r0_flux") and k.endswith("_in_rho")})
 
 
+@pytest.mark.parametrize("exchange", ["direct", "halo"])
 @pytest.mark.parametrize("case", PART_CASES)
-def test_gpu_partitioned_bc_flux_residual_dt_iterate(case, tmp_path):
-    say(f"[gpu] partitioned bc/flux/residual/dt/iterate case={case}")
+def test_gpu_partitioned_bc_flux_residual_dt_iterate(case, exchange, tmp_path, monkeypatch):
+    say(f"[gpu] partitioned bc/flux/residual/dt/iterate case={case} exchange={exchange}")
+    monkeypatch.setenv("E3D_EXCHANGE", exchange)
     g = load_golden(case)
     pd = integrator_params(g)
     m = composite(case, tmp_path)
     parts = m.partitions()
     nFI = m.info()["n_face_int"]
-    mpi = set(m.get("mpiGhost").tolist())
+    mpi = set(m.get("mpiGhost").tolist()) if exchange == "direct" else set()
     with EulerSolver(m, native_params(pd)) as s:
         keep = np.array([i for i in range(s.n_tot) if i not in mpi])
         for sd in seeds_of(g):
@@ -71,16 +78,18 @@ def test_gpu_partitioned_bc_flux_residual_dt_iterate(case, tmp_path):
             assert abs(sums[0] - ref) <= 1e-12 * abs(ref), (sums[0], ref)
 
 
-@pytest.mark.parametrize("path", ["fused", "fused256", "unfused"])
+@pytest.mark.parametrize("path", ["fused", "fused256", "unfused", "fused-halo", "unfused-halo"])
 @pytest.mark.parametrize("case", RUN_CASES)
 def test_gpu_partitioned_run_matches_reference(case, path, tmp_path, monkeypatch):
     """A whole multi-rank run on one device: final state of every partition bit for bit, the
     iteration count, the residual history and the coefficients summed per partition."""
     say(f"[gpu] partitioned run case={case} path={path}")
-    if path == "unfused":
+    if path.startswith("unfused"):
         monkeypatch.setenv("E3D_STAGE_PATH", "unfused")
     if path == "fused256":
         monkeypatch.setenv("E3D_BLOCK_CELLS", "256")
+    if path.endswith("halo"):
+        monkeypatch.setenv("E3D_EXCHANGE", "halo")
     g = load_golden(case)
     pd = integrator_params(g)
     m = composite(case, tmp_path)
@@ -99,14 +108,16 @@ def test_gpu_partitioned_run_matches_reference(case, path, tmp_path, monkeypatch
         assert abs(r[k] - ref) <= 1e-12 * max(abs(ref), 1e-12), (k, r[k], ref)
 
 
-def test_gpu_partitioned_ausm_matches_oracle(tmp_path):
+@pytest.mark.parametrize("exchange", ["direct", "halo"])
+def test_gpu_partitioned_ausm_matches_oracle(exchange, tmp_path, monkeypatch):
     """AUSM (no reference implementation) on partitioned input: device and oracle, both on the
-    composite mesh, agree bit for bit on the fluxes of a seeded state and over iterations (the
-    interface faces included)."""
+    composite mesh, agree bit for bit on the fluxes of a seeded state (after updateBC) and over
+    iterations (the interface faces included)."""
     from oracle_ctypes import Oracle
     from test_partitions import composite_arrays
 
-    say("[gpu] partitioned ausm case=bump_small_p4")
+    say(f"[gpu] partitioned ausm case=bump_small_p4 exchange={exchange}")
+    monkeypatch.setenv("E3D_EXCHANGE", exchange)
     g = load_golden("bump_small_p4")
     pd = integrator_params(g)
     m = composite("bump_small_p4", tmp_path)
@@ -117,6 +128,8 @@ def test_gpu_partitioned_ausm_matches_oracle(tmp_path):
         o = Oracle(arr, dict(pd, scheme=1))
         s.set_state(state)
         o.set_state(state)
+        s.update_bc()
+        o.update_bc()
         F, FS = s.face_flux()
         oF, oFS, _ = o.face_flux()
         assert bits_equal(F, oF) and bits_equal(FS, oFS)

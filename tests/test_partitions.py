@@ -390,3 +390,37 @@ def test_e3d_pre_executable(tmp_path):
             assert cm.info()["n_elem"] == gm.info()["n_elem"]
     r = subprocess.run([str(ROOT / "euler3d_amd" / "bin" / "E3D_PRE")], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "Usage" in r.stderr
+
+
+@pytest.mark.parametrize("case", PART_CASES)
+def test_interface_exchange_plan(case, tmp_path):
+    """The exchange lists (one message per sending/receiving partition pair, MPIHandler::
+    updateFlowField): every interface ghost receives exactly once, from the cell that the composite's
+    interface face reads (the pairing of the reference's two sorted lists, pinned by the fixtures
+    through that face); each message's cells belong to its sender and its ghosts to its receiver;
+    the two directions of a pair carry the same number of values, as MPI_Sendrecv requires."""
+    m = composite(case, tmp_path)
+    info = m.info()
+    ne, nfi = info["n_elem"], info["n_face_int"]
+    f2e = m.get("face2element").reshape(-1, 2)
+    start, frm, to = m.get("haloStart"), m.get("haloFrom"), m.get("haloTo")
+    src, dst = m.get("haloSrc"), m.get("haloDst")
+    mpi = m.get("mpiGhost")
+    part_start = m.get("partStart")
+    parts = m.partitions()
+    assert len(start) == len(frm) + 1 == len(to) + 1 and start[0] == 0 and start[-1] == len(src) == len(dst)
+    assert sorted(dst.tolist()) == sorted(mpi.tolist())
+    for g, c in zip(dst, src):
+        assert f2e[nfi + (g - ne), 1] == c
+    owner = lambda cell: int(np.searchsorted(part_start, cell, side="right") - 1)  # noqa: E731
+    sizes = {}
+    for k in range(len(frm)):
+        a, b = start[k], start[k + 1]
+        assert b > a
+        assert all(owner(c) == frm[k] for c in src[a:b])
+        r = parts[to[k]]
+        assert all(r["ghost0"] <= g - ne < r["ghost0"] + r["n_ghost"] for g in dst[a:b])
+        sizes[(int(frm[k]), int(to[k]))] = b - a
+    for (s, r), n in sizes.items():
+        if s != r:
+            assert sizes[(r, s)] == n, (s, r)
