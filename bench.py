@@ -423,6 +423,15 @@ def main() -> None:
               f"--gpus {a.gpus} is not supported (multi-GPU sharding is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
+    if a.copy_calib_mib:
+        import ctypes
+
+        from euler3d_amd import native
+
+        ms = ctypes.c_double()
+        native.check(native.lib().e3d_copy_probe(0, a.copy_calib_mib << 20, ctypes.byref(ms)))
+        log(f"copy calibration: {a.copy_calib_mib} MiB read + {a.copy_calib_mib} MiB written in {ms.value:.3f} ms")
+
     if a.mode == "batch":
         print(json.dumps(bench_batch(a)), flush=True)
         return
@@ -434,15 +443,6 @@ def main() -> None:
         return
 
     from euler3d_amd import EulerSolver, Mesh, default_params
-
-    if a.copy_calib_mib:
-        import ctypes
-
-        from euler3d_amd import native
-
-        ms = ctypes.c_double()
-        native.check(native.lib().e3d_copy_probe(0, a.copy_calib_mib << 20, ctypes.byref(ms)))
-        log(f"copy calibration: {a.copy_calib_mib} MiB read + {a.copy_calib_mib} MiB written in {ms.value:.3f} ms")
 
     t0 = time.perf_counter()
     mesh = Mesh.bump(nx=a.nx, ny=a.ny, nz=a.nz, jitter=0.1, seed=0, permute=True)
