@@ -107,7 +107,7 @@ struct e3d_ctx {
 	int nPart = 1;
 	std::vector<int> wall0, nWallPart;
 	std::vector<int> mpiGhost, mpiCell;
-	DBuf<double> aeroOut;
+	DBuf<double> aeroOut, aeroTmp; // aeroTmp: chunk sums of the force integration
 	DBuf<double> wallCp;  // cp of every wall face at the last force update (AeroCoefficients::_cp)
 	bool wallCpSet = false;
 	// partitioned input with the interface exchange (E3D_EXCHANGE=halo): interface faces read their
@@ -752,6 +752,9 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		c->nWall = (int) P.wallCell.size();
 		if (c->nWall) {
 			c->wallCp.alloc(c->nWall);
+			int most = 0;
+			for (int w : c->nWallPart) most = std::max(most, w);
+			c->aeroTmp.alloc(e3d::aeroScratch(most));
 			c->wallCell.alloc(c->nWall), c->wallCell.upload(P.wallCell.data(), c->nWall, s);
 			for (int k = 0; k < 3; k++) {
 				c->wN[k].alloc(c->nWall), c->wN[k].upload(P.wallN[k].data(), c->nWall, s);
@@ -1013,7 +1016,8 @@ int e3d_aero(e3d_ctx *c, double cf[3], double cm[3]) {
 			e3d::launchAero(c->nWallPart[r], c->wallCell.p + w0, c->wN[0].p + w0, c->wN[1].p + w0, c->wN[2].p + w0,
 			                c->wC[0].p + w0, c->wC[1].p + w0, c->wC[2].p + w0, c->state(),
 			                c->wMode ? c->wst[c->cur].p : nullptr, c->fr.gamma, c->cpFactor, c->fr.p_inf,
-			                c->prm.cref, c->prm.sref, c->aeroOut.p + 6 * (size_t) r, c->wallCp.p + w0, c->stream);
+			                c->prm.cref, c->prm.sref, c->aeroTmp.p, c->aeroOut.p + 6 * (size_t) r, c->wallCp.p + w0,
+			                c->stream);
 			c->check();
 			any = true;
 		}

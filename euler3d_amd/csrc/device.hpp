@@ -152,11 +152,13 @@ int sumScratch(int n);
 void launchSum(const double *in, int n, double *tmp, double *out, hipStream_t st);
 // interior Mach number from the primitives (FlowField.cpp:157), for state read-back
 void launchMach(const DevMesh &m, DevState s, const DevFree &q, hipStream_t st);
-// wall-face force/moment: out[6] = (Fx,Fy,Fz, CPx,CPy,CPz) summed in a fixed order
-// (wall cells' p from W when W is not null, else from the primitive records of s); cpOut (may be
-// null) receives each wall face's cp
+// wall-face force/moment: out[6] = (Fx,Fy,Fz, CPx,CPy,CPz) summed in a fixed order over several
+// workgroups (wall cells' p from W when W is not null, else from the primitive records of s);
+// cpOut (may be null) receives each wall face's cp; tmp holds aeroScratch(nWall) doubles
+int aeroScratch(int nWall);
 void launchAero(int nWall, const int *wallCell, const double *Nx, const double *Ny, const double *Nz,
                 const double *Cx, const double *Cy, const double *Cz, DevState s, const double *W, double gamma,
-                double cpFactor, double pInf, double cref, double sref, double *out6, double *cpOut, hipStream_t st);
+                double cpFactor, double pInf, double cref, double sref, double *tmp, double *out6, double *cpOut,
+                hipStream_t st);
 
 }// namespace e3d

@@ -1100,15 +1100,23 @@ __global__ void __launch_bounds__(kBlock) kMach(DevMesh m, DevState s, DevFree q
 
 // AeroCoefficients::Update (AeroCoefficients.cpp:38-52): per wall face coeff = N*cp*Cref/Sref and
 // coeff*x_face; summed per thread over a contiguous chunk, then a fixed tree
-__global__ void __launch_bounds__(kBlock) kAero(int nWall, const int *wallCell, const double *Nx, const double *Ny,
-                                                const double *Nz, const double *Cx, const double *Cy, const double *Cz,
-                                                DevState s, const double *W, double gamma, double cpFactor, double pInf,
-                                                double cref, double sref, double *out6, double *cpOut) {
-	__shared__ double lds[kBlock];
+// AeroCoefficients::Update (AeroCoefficients.cpp:38-52) over the wall faces, in two launches over
+// several workgroups: kAeroChunks sums chunk g of 4 kBlock faces (each thread its four faces t,
+// t + kBlock, ... in that order, then groupSum) into tmp[6 g + k]; kAeroTotal adds the chunk sums
+// the same way. The wall cells' p comes from W when W is not null (as the update forms it), else
+// from the primitive records; cpOut (may be null) receives each face's cp.
+constexpr int kAeroChunk = 4 * kBlock;
+__global__ void __launch_bounds__(kBlock) kAeroChunks(int nWall, const int *wallCell, const double *Nx,
+                                                      const double *Ny, const double *Nz, const double *Cx,
+                                                      const double *Cy, const double *Cz, DevState s, const double *W,
+                                                      double gamma, double cpFactor, double pInf, double cref,
+                                                      double sref, double *tmp, double *cpOut) {
+	__shared__ double lds[kBlock / 64];
 	double acc[6] = {0, 0, 0, 0, 0, 0};
-	int per = (nWall + kBlock - 1) / kBlock;
-	int b = threadIdx.x * per, en = min(nWall, b + per);
-	for (int i = b; i < en; i++) {
+#pragma unroll
+	for (int j = 0; j < 4; j++) {
+		const int i = blockIdx.x * kAeroChunk + j * kBlock + threadIdx.x;
+		if (i >= nWall) break;
 		double pw;
 		if (W) {
 			// p of the conservative state, as the update forms it
@@ -1131,7 +1139,17 @@ __global__ void __launch_bounds__(kBlock) kAero(int nWall, const int *wallCell, 
 		acc[5] += cz * Cz[i];
 	}
 	for (int k = 0; k < 6; k++) {
-		double t = blockSum(acc[k], lds);
+		const double t = groupSum<kBlock>(acc[k], lds);
+		if (threadIdx.x == 0) tmp[6 * blockIdx.x + k] = t;
+		__syncthreads();
+	}
+}
+__global__ void __launch_bounds__(kBlock) kAeroTotal(const double *tmp, int chunks, double *out6) {
+	__shared__ double lds[kBlock / 64];
+	for (int k = 0; k < 6; k++) {
+		double acc = 0.0;
+		for (int g = threadIdx.x; g < chunks; g += kBlock) acc += tmp[6 * g + k];
+		const double t = groupSum<kBlock>(acc, lds);
 		if (threadIdx.x == 0) out6[k] = t;
 		__syncthreads();
 	}
@@ -1392,11 +1410,15 @@ void launchMach(const DevMesh &m, DevState s, const DevFree &q, hipStream_t st) 
 	hipLaunchKernelGGL(kMach, dim3(blocksFor(m.nElem)), dim3(kBlock), 0, st, m, s, q);
 }
 
+int aeroScratch(int nWall) { return 6 * std::max(1, (nWall + kAeroChunk - 1) / kAeroChunk); }
+
 void launchAero(int nWall, const int *wallCell, const double *Nx, const double *Ny, const double *Nz, const double *Cx,
                 const double *Cy, const double *Cz, DevState s, const double *W, double gamma, double cpFactor,
-                double pInf, double cref, double sref, double *out6, double *cpOut, hipStream_t st) {
-	hipLaunchKernelGGL(kAero, dim3(1), dim3(kBlock), 0, st, nWall, wallCell, Nx, Ny, Nz, Cx, Cy, Cz, s, W, gamma,
-	                   cpFactor, pInf, cref, sref, out6, cpOut);
+                double pInf, double cref, double sref, double *tmp, double *out6, double *cpOut, hipStream_t st) {
+	const int chunks = std::max(1, (nWall + kAeroChunk - 1) / kAeroChunk);
+	hipLaunchKernelGGL(kAeroChunks, dim3(chunks), dim3(kBlock), 0, st, nWall, wallCell, Nx, Ny, Nz, Cx, Cy, Cz, s, W,
+	                   gamma, cpFactor, pInf, cref, sref, tmp, cpOut);
+	hipLaunchKernelGGL(kAeroTotal, dim3(1), dim3(kBlock), 0, st, tmp, chunks, out6);
 }
 
 }// namespace e3d
