@@ -218,6 +218,7 @@ def bench_batch(a) -> dict:
             if all(s["done"] for s in st):
                 break
         wall = time.perf_counter() - t0
+        outcome = {s["id"]: s["outcome"] for s in b.poll()}
         res = [b.result(i) for i in ids]
         iters = [len(r["rms"]) for r in res]
     cell_stages = 5 * sum(c * i for c, i in zip(cells, iters))
@@ -237,8 +238,11 @@ def bench_batch(a) -> dict:
                    "cells_total": sum(cells), "launch_shape": list(tune["best"])},
         "wall_s": wall, "admission_s": t_admit, "cell_stage_per_s": cell_stages / wall,
         "iterations": {"min": min(iters), "max": max(iters), "mean": sum(iters) / len(iters)},
+        "outcomes": {k: sum(1 for i in ids if outcome[i] == k) for k in ("converged", "capped", "failed")},
         "plain_fp64": {"wall_s": wall_plain, "solves_per_h": 3600.0 * len(cases) / wall_plain,
-                       "iterations_mean": sum(q["iterations"] for q in plain) / len(plain)},
+                       "iterations_mean": sum(q["iterations"] for q in plain) / len(plain),
+                       "outcomes": {k: sum(1 for q in plain if q["outcome"] == k)
+                                    for k in ("converged", "capped", "failed")}},
         "speedup_vs_plain": wall_plain / wall,
         "max_abs_dCL_vs_fp64": dcl, "max_abs_dCD_vs_fp64": dcd,
         "autotune": tune["table"],
@@ -273,7 +277,8 @@ def bench_serve(a) -> list[dict]:
             sv = summarize(rec)
             lat, span = fifo_latencies(arr, services)
             pl = {"p50_s": percentile(lat, 50), "p99_s": percentile(lat, 99), "makespan_s": span,
-                  "solves_per_h": 3600.0 * len(cases) / span}
+                  "solves_per_h": 3600.0 * len(cases) / span,
+                  **{k: sum(1 for q in plain if q["outcome"] == k) for k in ("converged", "capped", "failed")}}
             dcl = max(abs(r["CL"] - q["CL"]) for r, q in zip(rec, plain))
             dcd = max(abs(r["CD"] - q["CD"]) for r, q in zip(rec, plain))
             line = {"metric": "served solves/h", "value": sv["solves_per_h"], "unit": "solves/h", "n_gpus": 1,
@@ -285,7 +290,9 @@ def bench_serve(a) -> list[dict]:
                     "max_abs_dCL_vs_fp64": dcl, "max_abs_dCD_vs_fp64": dcd,
                     "requests": [{"kind": cases[i].kind, "cells": cells[i], "arrival": round(r["arrival"], 4),
                                   "admitted": round(r.get("admitted", float("nan")), 4),
-                                  "finished": round(r["finished"], 4), "iterations": r["iterations"],
+                                  "finished": round(r["finished"], 4), "iterations": r.get("iterations"),
+                                  "outcome": r.get("outcome"), "plain_iterations": plain[i]["iterations"],
+                                  "plain_outcome": plain[i]["outcome"],
                                   "plain_service": round(plain[i]["service"], 4)} for i, r in enumerate(rec)]}
             log(f"rate {lam}: served p50 {sv['p50_s']:.2f} s p99 {sv['p99_s']:.2f} s {sv['solves_per_h']:.0f}/h; "
                 f"plain p50 {pl['p50_s']:.2f} s p99 {pl['p99_s']:.2f} s {pl['solves_per_h']:.0f}/h")
