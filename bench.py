@@ -269,7 +269,8 @@ def bench_serve(a) -> list[dict]:
         log(f"plain {i}: {cases[i].kind} {cells[i]} cells, {plain[i]['service']:.2f} s, {plain[i]['iterations']} it")
     services = [q["service"] for q in plain]
     lines = []
-    srv = Server(drop=a.drop, max_iter=a.max_iter, hbm_budget=200e9, admit_workers=a.admit_workers)
+    srv = Server(drop=a.drop, max_iter=a.max_iter, hbm_budget=200e9, admit_workers=a.admit_workers,
+                 **({"stall": None} if a.no_stall else {}))
     try:
         for lam in a.lambdas:
             arr = poisson_arrivals(len(cases), lam, seed=0)
@@ -285,13 +286,16 @@ def bench_serve(a) -> list[dict]:
                     "higher_is_better": True, "dtype": "f32 flux / f64 update", "data": "synthetic",
                     "config": {"workload": f"{len(cases)} requests (naca/bump/wing, 10k-{a.max_cells:.0f} cells, "
                                            f"seed 0), Poisson rate {lam} req/s, {a.drop:g} drop or {a.max_iter} it, "
-                                           f"200 GB HBM budget", "rate": lam, "cells_total": sum(cells)},
+                                           f"200 GB HBM budget", "rate": lam, "cells_total": sum(cells),
+                               "stall_test": None if a.no_stall else list(srv.stall)},
                     "served": sv, "plain_fp64": pl, "vs_plain_solves_per_h": sv["solves_per_h"] / pl["solves_per_h"],
                     "max_abs_dCL_vs_fp64": dcl, "max_abs_dCD_vs_fp64": dcd,
                     "requests": [{"kind": cases[i].kind, "cells": cells[i], "arrival": round(r["arrival"], 4),
                                   "admitted": round(r.get("admitted", float("nan")), 4),
                                   "finished": round(r["finished"], 4), "iterations": r.get("iterations"),
-                                  "outcome": r.get("outcome"), "plain_iterations": plain[i]["iterations"],
+                                  "outcome": r.get("outcome"), "stalled_at": r.get("stalled_at"),
+                                  "fp64_iterations": r.get("fp64_iterations"),
+                                  "plain_iterations": plain[i]["iterations"],
                                   "plain_outcome": plain[i]["outcome"],
                                   "plain_service": round(plain[i]["service"], 4)} for i, r in enumerate(rec)]}
             log(f"rate {lam}: served p50 {sv['p50_s']:.2f} s p99 {sv['p99_s']:.2f} s {sv['solves_per_h']:.0f}/h; "
@@ -331,7 +335,7 @@ def served_leg(a) -> dict:
             "workload": f"6 requests (naca/bump/wing, {min(cells)}-{max(cells)} cells, seed 0), Poisson 20 req/s, "
                         "1e-6 drop or 5000 iterations, 200 GB HBM budget",
             "served": {"p50_s": sv["p50_s"], "p99_s": sv["p99_s"], "makespan_s": sv["makespan_s"],
-                       "solves_per_h": sv["solves_per_h"], **outcomes(rec)},
+                       "solves_per_h": sv["solves_per_h"], **outcomes(rec), "finished_fp64": sv["finished_fp64"]},
             "plain_fp64": {"p50_s": percentile(lat, 50), "p99_s": percentile(lat, 99), "makespan_s": span,
                            "solves_per_h": 3600.0 * len(cases) / span, **outcomes(plain)},
             "vs_plain_solves_per_h": sv["solves_per_h"] / (3600.0 * len(cases) / span),
@@ -424,6 +428,8 @@ def main() -> None:
     ap.add_argument("--drop", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=5000)
     ap.add_argument("--admit-workers", type=int, default=8)
+    ap.add_argument("--no-stall", action="store_true",
+                    help="serve without the stall test (stalled cases are not finished on the FP64 path)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs one resident mesh per GPU and measures one GPU only; "

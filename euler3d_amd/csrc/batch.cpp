@@ -388,6 +388,9 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 		return fail(E3D_ERR_UNSUPPORTED, "the mixed-precision engine runs RK5 (TIME_INTEGRATION= 2) without residual "
 		                                 "smoothing; use e3d_create for the other integrators");
 	if (co->max_iter <= 0 || co->hist_cap < 0 || co->drop < 0) return fail(E3D_ERR_ARG, "invalid case options");
+	if (co->stall_window < 0 ||
+	    (co->stall_window > 0 && !(co->stall_ratio > 0 && co->stall_ratio <= 1 && co->stall_level > 0)))
+		return fail(E3D_ERR_ARG, "invalid stall test (stall_window >= 0, 0 < stall_ratio <= 1, stall_level > 0)");
 	if (!m->parts.empty())
 		return fail(E3D_ERR_UNSUPPORTED, "the mixed-precision engine takes single meshes; use e3d_create for partitioned input");
 	const e3d::SolverMesh &sm = m->sm;
@@ -546,6 +549,9 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 		d.maxIter = co->max_iter;
 		d.drop = co->drop;
 		d.minResidual = p->min_residual;
+		d.stallWindow = co->stall_window;
+		d.stallRatio = co->stall_ratio;
+		d.stallLevel = co->stall_level;
 		d.blockSlot = (const int *) at(0);
 		d.blockInt = (const int *) at(23);
 		d.blockHalo = (const int *) at(1);

@@ -486,7 +486,11 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 }
 
 // one workgroup per case slot: RMS of the last stage (fixed-order sum of the block partials),
-// residual history, convergence and buffer parity
+// residual history, convergence, the stall test and buffer parity.
+// Stall test (stallWindow > 0): iterations are cut into windows of stallWindow from the first; at
+// the end of a window its lowest rms is compared with the previous window's. A case whose lowest rms
+// has stopped falling (not below stallRatio x the previous) while below stallLevel x the first rms
+// retires as stalled: the FP32 fluxes' rounding floor, which the FP64 path can take further.
 constexpr int kEndThreads = 1024;
 __global__ void __launch_bounds__(kEndThreads) kMixIterEnd(const MixCase *cases, MixStatus *status) {
 	__shared__ double red[kEndThreads / 64];
@@ -509,9 +513,20 @@ __global__ void __launch_bounds__(kEndThreads) kMixIterEnd(const MixCase *cases,
 		// a NaN or infinite residual retires the case as failed; it never counts as converged
 		const bool bad = !(rms - rms == 0.0);
 		const bool conv = (C.drop > 0.0 && rms < C.drop * r0) || rms < C.minResidual;
+		bool stalled = false;
+		if (C.stallWindow > 0) {
+			const double wm = it == st.winStart ? rms : fmin(st.winMin, rms);
+			st.winMin = wm;
+			if (it + 1 - st.winStart == C.stallWindow) {
+				stalled = st.prevMin > 0.0 && !(wm < C.stallRatio * st.prevMin) && wm < C.stallLevel * r0;
+				st.prevMin = wm;
+				st.winStart = it + 1;
+			}
+		}
 		if (bad) st.done = 3;
 		else if (conv) st.done = 1;
 		else if (it + 1 >= C.maxIter) st.done = 2;
+		else if (stalled) st.done = 4;
 	}
 }
 

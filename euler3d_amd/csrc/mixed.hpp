@@ -10,8 +10,10 @@
 // (d_rho, d_u, d_v, d_w, d_p, d_H) and each face contributes F - F_inf(n) instead of F, with
 // F_inf(n) the freestream flux through the face. The subtracted terms sum to F_inf . sum(n S) = 0
 // over a closed cell, so the residual is unchanged in exact arithmetic, while the FP32 rounding
-// error scales with the local departure from freestream instead of with |F|. That keeps the
-// residual floor of the FP32 flux far below the 1e-6 drop the cases converge to.
+// error scales with the local departure from freestream instead of with |F|. The residual still
+// flattens where the FP32 rounding of those perturbations dominates, typically 5.5 to 6.5 orders
+// below the first residual; the stall test below retires a case there so that the FP64 path can
+// finish it.
 //
 // Cases. Every resident case owns its arrays (plan, metrics, W, two FP32 record buffers,
 // per-block partial sums, residual history). A device table of MixCase descriptors and a table
@@ -38,9 +40,12 @@ struct MixCase {
 	int scheme; // 0 Roe, 1 AUSM
 	int histCap;
 	int maxIter;
-	int pad0;
+	int stallWindow;    // iterations per window of the stall test, 0: off
 	double drop;        // converged when rms < drop * rms of the first iteration
 	double minResidual; // or when rms < minResidual (absolute)
+	// stalled when a window's lowest rms is not below stallRatio x the previous window's lowest and
+	// below stallLevel x the first rms: the residual has reached the floor of the FP32 fluxes
+	double stallRatio, stallLevel;
 	// block plan: each block's slots list its interior faces first (blockInt of them), then its
 	// boundary faces
 	const int *blockSlot, *blockInt, *blockHalo, *haloCell, *ghostOf;
@@ -68,9 +73,11 @@ struct MixStatus {
 	int parity; // FP32 record buffer holding the state at the start of an iteration
 	int iter;   // iterations done
 	int done;   // nonzero: finished, its blocks are skipped. 1 converged, 2 iteration cap, 3 non-finite
-	            // residual (E3D_CASE_CONVERGED, _CAPPED, _FAILED)
+	            // residual, 4 stalled (E3D_CASE_CONVERGED, _CAPPED, _FAILED, _STALLED)
 	int active; // 0: the slot holds no case
 	double rms0, rmsLast;
+	int winStart, pad; // first iteration of the current stall-test window
+	double winMin, prevMin; // lowest rms of the current and of the previous window (0: none yet)
 };
 
 // launch choices of the mixed stage (they change speed only, never results)
@@ -99,7 +106,7 @@ size_t mixLdsBytes(int maxSlots, int maxHalo);
 int mixResidentPerCU(int ms, int cap, int schemes, bool nontemporal);
 // one RK stage (0..4) of every active, unfinished case
 void launchMixStage(const MixLaunch &L, int stage, const MixTuning &t, hipStream_t st);
-// after the 5 stages: per-case RMS, history, convergence, buffer parity
+// after the 5 stages: per-case RMS, history, convergence and stall tests, buffer parity
 void launchMixIterEnd(const MixLaunch &L, hipStream_t st);
 // FP64 primitives (rho, u, v, w, p, E, H, M) of the interior cells from W: out[8][nElem]
 // wall and symmetry ghost records of the last completed iteration into rec[parity]

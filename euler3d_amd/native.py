@@ -19,7 +19,7 @@ E3D_ERR_ARG = 1
 E3D_ERR_BUSY = 6
 E3D_ERR_NUMERIC = 7
 # done codes of a batched case (E3D_CASE_* in include/e3d.h)
-CASE_OUTCOMES = {0: "running", 1: "converged", 2: "capped", 3: "failed"}
+CASE_OUTCOMES = {0: "running", 1: "converged", 2: "capped", 3: "failed", 4: "stalled"}
 
 
 class E3DError(RuntimeError):
@@ -56,7 +56,8 @@ class BatchOpts(C.Structure):
 
 
 class CaseOpts(C.Structure):
-    _fields_ = [("drop", C.c_double), ("max_iter", C.c_int), ("hist_cap", C.c_int)]
+    _fields_ = [("drop", C.c_double), ("max_iter", C.c_int), ("hist_cap", C.c_int), ("stall_window", C.c_int),
+                ("stall_ratio", C.c_double), ("stall_level", C.c_double)]
 
 
 _lib = None

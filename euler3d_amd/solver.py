@@ -316,11 +316,17 @@ class Batch:
         self._n = {}
 
     @staticmethod
-    def case_opts(drop: float = 1e-6, max_iter: int = 5000, hist_cap: int | None = None) -> N.CaseOpts:
+    def case_opts(drop: float = 1e-6, max_iter: int = 5000, hist_cap: int | None = None,
+                  stall: tuple | None = None) -> N.CaseOpts:
+        """stall: (window, ratio, level) of the stall test, None: off. The case retires as "stalled"
+        when the lowest RMS of a window of `window` iterations is not below ratio x the previous
+        window's lowest while below level x the first RMS."""
         co = N.CaseOpts()
         co.drop = float(drop)
         co.max_iter = int(max_iter)
         co.hist_cap = int(max_iter if hist_cap is None else hist_cap)
+        if stall is not None:
+            co.stall_window, co.stall_ratio, co.stall_level = int(stall[0]), float(stall[1]), float(stall[2])
         return co
 
     @staticmethod
@@ -331,9 +337,10 @@ class Batch:
         return b.value
 
     def add(self, mesh: Mesh, params: N.Params, drop: float = 1e-6, max_iter: int = 5000,
-            hist_cap: int | None = None) -> int:
-        """Admit a case; returns its id. Raises E3DError with code E3D_ERR_BUSY when full."""
-        co = self.case_opts(drop, max_iter, hist_cap)
+            hist_cap: int | None = None, stall: tuple | None = None) -> int:
+        """Admit a case; returns its id. Raises E3DError with code E3D_ERR_BUSY when full, E3D_ERR_ARG
+        when it can never fit. stall: see case_opts."""
+        co = self.case_opts(drop, max_iter, hist_cap, stall)
         cid = C.c_int()
         N.check(N.lib().e3d_batch_add(self._h, mesh._h, C.byref(params), C.byref(co), C.byref(cid)))
         info = mesh.info()
@@ -347,7 +354,8 @@ class Batch:
         N.check(N.lib().e3d_batch_wait(self._h))
 
     def status(self, cid: int) -> dict:
-        """done: finished; outcome: running / converged / capped / failed (non-finite residual)."""
+        """done: finished; outcome: running / converged / capped / failed (non-finite residual) /
+        stalled (residual at the FP32 flux floor; finish on the FP64 path)."""
         done, it = C.c_int(), C.c_int()
         r0, r1 = C.c_double(), C.c_double()
         rc = N.lib().e3d_batch_status(self._h, int(cid), C.byref(done), C.byref(it), C.byref(r0), C.byref(r1))

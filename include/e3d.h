@@ -265,11 +265,20 @@ typedef struct e3d_batch_opts {
 #define E3D_CASE_CONVERGED 1 /* residual drop or MIN_RESIDUAL reached */
 #define E3D_CASE_CAPPED 2    /* iteration cap reached first */
 #define E3D_CASE_FAILED 3    /* residual NaN or infinite: retired, the other cases are unaffected */
+#define E3D_CASE_STALLED 4   /* residual stopped falling at the FP32 flux floor before the drop (stall
+                                test of e3d_case_opts); finish on the FP64 path: e3d_batch_get_state,
+                                e3d_create, e3d_set_state, e3d_iterate */
 
 typedef struct e3d_case_opts {
 	double drop;         /* converged when rms < drop * rms of the first iteration (0: off) */
 	int max_iter;        /* iteration cap */
 	int hist_cap;        /* residual RMS history entries kept on the device */
+	int stall_window;    /* stall test, 0: off. Iterations are cut into windows of stall_window; a
+	                        case retires as E3D_CASE_STALLED when a window's lowest RMS is not below
+	                        stall_ratio x the previous window's lowest and is below stall_level x the
+	                        first RMS */
+	double stall_ratio;  /* (0, 1] */
+	double stall_level;  /* > 0 */
 } e3d_case_opts;
 
 int e3d_batch_opts_default(e3d_batch_opts *out);
