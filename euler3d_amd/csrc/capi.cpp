@@ -80,7 +80,7 @@ struct e3d_ctx {
 	DBuf<double> snx, sny, snz, sS;
 	DBuf<uint16_t> csr;
 	DBuf<uint32_t> spack;
-	DBuf<int> blockHalo, haloCell;
+	DBuf<int> blockHalo, haloCell, blockMeta;
 	// state: two buffers of (nElem + nGhost) x 8 doubles, st[cur] is the current one
 	DBuf<double> st[2];
 	int cur = 0;
@@ -738,7 +738,32 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		c->haloCell.alloc(std::max<size_t>(1, P.haloCell.size())), c->haloCell.upload(P.haloCell.data(), P.haloCell.size(), s);
 		c->plan = {P.nBlocks, P.maxSlots, P.csrStride, c->blockSlot.p, c->se0.p, c->se1.p, c->skind.p,
 		           c->snx.p, c->sny.p, c->snz.p, c->sS.p, c->csr.p, c->spack.p, c->blockHalo.p, c->haloCell.p,
-		           P.maxHalo, P.cellsPerBlock};
+		           P.maxHalo, P.cellsPerBlock, nullptr};
+		{
+			std::vector<int> meta(4 * (size_t) P.nBlocks);
+			for (int b = 0; b < P.nBlocks; b++) {
+				int nInt = 0;
+				while (P.blockSlot[b] + nInt < P.blockSlot[b + 1] && P.skind[P.blockSlot[b] + nInt] == 0) nInt++;
+				meta[4 * (size_t) b] = P.blockSlot[b];
+				meta[4 * (size_t) b + 1] = (P.blockSlot[b + 1] - P.blockSlot[b]) | nInt << 16;
+				meta[4 * (size_t) b + 2] = P.blockHalo[b];
+				meta[4 * (size_t) b + 3] = P.blockHalo[b + 1] - P.blockHalo[b];
+			}
+			c->blockMeta.alloc(std::max<size_t>(4, meta.size()));
+			c->blockMeta.upload(meta.data(), meta.size(), s);
+			c->plan.blockMeta = reinterpret_cast<const int4 *>(c->blockMeta.p);
+		}
+		// The persistent, software-pipelined stage where a block's LDS leaves room for one
+		// workgroup per CU (there no other workgroup hides a block's load phase; on the 1.18M-hex
+		// wing 135 against 157 us per stage). Where several workgroups fit, they hide each other's
+		// loads better than the pipeline's two (6.3M-tet bump: 381 against 437 us). E3D_PIPE=0|1
+		// turns it off or on for every plan it fits.
+		{
+			const char *v = std::getenv("E3D_PIPE");
+			const bool wideLds = 2 * e3d::stageFusedLdsBytes(P.cellsPerBlock, P.maxSlots, P.maxHalo, 2) > 160 * 1024;
+			const bool want = v ? std::atoi(v) != 0 : wideLds;
+			if (want) c->tune.pipeGrid = e3d::stagePipeGrid(c->plan, p->scheme, c->device, c->tune.xcds);
+		}
 		// partitions: wall ranges, interface ghosts
 		c->nPart = sm.partStart.size() > 1 ? (int) sm.partStart.size() - 1 : 1;
 		if (!m->ranges.empty()) {

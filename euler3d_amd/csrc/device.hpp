@@ -74,6 +74,9 @@ struct DevPlan {
 	const int *haloCell;     // outside cells read by each block, device ids
 	int maxHalo;
 	int cellsPerBlock;       // 256, or 128 where a 256-cell block needs too much LDS
+	// per block: first slot, slot count | leading interior (kind 0) slots << 16, first halo entry,
+	// halo count
+	const int4 *blockMeta;
 };
 
 void launchBc(const DevMesh &m, DevState s, const DevFree &q, int machDerived, hipStream_t st);
@@ -99,6 +102,9 @@ int stageBlocks(int nElem);
 size_t stageFusedLdsBytes(int cellsPerBlock, int maxSlots, int maxHalo, int layout);
 // waves the fused RK stage keeps resident per CU for such a plan (0: a block does not fit)
 int stageFusedWavesPerCU(int cellsPerBlock, int maxSlots, int maxHalo, int scheme);
+// workgroups of the persistent, software-pipelined stage for this plan (every one resident at
+// once), or 0 when the plan runs the one-block-per-workgroup kernel
+int stagePipeGrid(const DevPlan &p, int scheme, int device, int xcds);
 // launch choices of the fused stage (they change speed only, never results)
 struct StageTuning {
 	bool nontemporal = true; // non-temporal loads for arrays read once per stage
@@ -109,6 +115,8 @@ struct StageTuning {
 	// every other stage walks each XCD's block range backwards, so that it starts on the cells
 	// the previous stage wrote last, which are still in the memory-side cache
 	bool alternate = true;
+	// > 0: the persistent, software-pipelined stage with this many workgroups (stagePipeGrid)
+	int pipeGrid = 0;
 };
 
 // Fused RK stage: boundary ghosts, face fluxes (once per block, in LDS), residual gather, local

@@ -1024,6 +1024,341 @@ __global__ void __launch_bounds__(NTH) kStageFused(DevMesh m, DevPlan p, DevStat
 	}
 }
 
+// Persistent, software-pipelined form of the fused RK stage (conservative states in and out, RK
+// stages): a grid of as many workgroups as fit on the device at once, each walking a list of
+// blocks, with the loads of the blocks ahead in flight in registers while a block computes (see
+// the loop below). It serves plans whose blocks leave LDS for one workgroup per CU, where no other
+// workgroup hides a block's load phase. The arithmetic of a block is kStageFused's, statement for
+// statement, so the results are the same bits. KH: halo cells per thread held in flight (blocks' halos <= KH * NTH); KS: rounds of
+// slot records held (blocks' slots <= KS * NTH). A workgroup's blocks are its XCD's range of the
+// block order (as logicalBlock lays it out), strided by the XCD's workgroup count, backwards when
+// `reverse` is set; gridDim.x is a multiple of xcds.
+template<int SCHEME, bool FIXED, int NTH, int KH, int KS>
+__global__ void __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(2)))
+kStagePipe(DevMesh m, DevPlan p, DevState in, DevState out, const double *Win, double *Wout, DevFree q, double *W0,
+           double alpha, int firstStage, double *rmsPartial, int xcds, int reverse) {
+	constexpr int CB = 256;
+	constexpr bool NT = true, MK = true;
+	constexpr int kCons = (5 * CB / 2 + NTH - 1) / NTH;
+	extern __shared__ double lds[];
+	const int t = threadIdx.x;
+	const int cap = FIXED ? kFixCap : CB + p.maxHalo;
+	const int ms = FIXED ? kFixSlots : p.maxSlots;
+	// LDS as kStageFused with conservative states: [6][cap] rho, u, v, w, p, H; [cap] sqrt(rho);
+	// [cap] sound speed; [5][ms] F*S per slot, which first stages the block's own W
+	double *ls = lds, *lsq = ls + 6 * cap, *lcs = lsq + cap, *lf = lcs + cap;
+	const size_t n = (size_t) m.nElem;
+
+	// this workgroup's blocks
+	const int X = xcds > 1 ? xcds : 1;
+	const int x = (int) blockIdx.x % X, k0 = (int) blockIdx.x / X, perX = (int) gridDim.x / X;
+	const int qq = p.nBlocks / X, rr = p.nBlocks % X;
+	const int len = qq + (x < rr ? 1 : 0), lo = x * qq + min(x, rr);
+	if (k0 >= perX) return; // a grid that is not a multiple of xcds: the surplus workgroups idle
+	auto blockAt = [&](int i) -> int {
+		const int j = k0 + i * perX;
+		if (j >= len) return -1;
+		return reverse ? lo + len - 1 - j : lo + j;
+	};
+
+	// Loads in flight, three blocks deep. While block i computes: the own and halo states of block
+	// i + 1 (issued before i's flux loop, from halo ids loaded during block i - 1), the halo ids of
+	// block i + 2, the meta of block i + 3, and, after i's flux loop, block i + 1's slot records,
+	// W0, metrics and CSR codes (which its flux loop and update need later). No load waits for
+	// another load issued in the same block.
+	double2 nW[kCons];
+	double nw0[5] = {0, 0, 0, 0, 0}, nvol = 0, nsx = 0, nsy = 0, nsz = 0;
+	// the CSR codes of the own cell as loaded, two per word (decoded where the gather uses them)
+	uint32_t ncw[3];
+	int aIdx[KH];
+	double nHW[KH][5];
+	uint32_t nPk[KS];
+	double nNx[KS], nNy[KS], nNz[KS], nS[KS];
+	auto uni = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+	// halo ids of block b (meta mb: first slot, slots | leading interior slots << 16, first halo
+	// entry, halo cells)
+	auto issueIds = [&](int b, int4 mb) {
+		const int h0 = uni(mb.z), nh = b >= 0 ? uni(mb.w) : 0;
+#pragma unroll
+		for (int i = 0; i < KH; i++) {
+			const int h = i * NTH + t;
+			aIdx[i] = h < nh ? p.haloCell[h0 + h] : -1;
+		}
+	};
+	// own and halo states of block b (its halo ids in aIdx)
+	auto issueW = [&](int b) {
+		if (b < 0) return;
+		const int c0 = b * CB, nc = min(CB, m.nElem - c0);
+		const double2 *blk = reinterpret_cast<const double2 *>(Win + (size_t) 5 * c0);
+#pragma unroll
+		for (int i = 0; i < kCons; i++) {
+			const int j = i * NTH + t;
+			nW[i] = 2 * j < 5 * nc ? blk[j] : make_double2(0, 0);
+		}
+#pragma unroll
+		for (int i = 0; i < KH; i++) {
+			if (aIdx[i] >= 0) {
+				const double *hw = Win + (size_t) 5 * aIdx[i];
+#pragma unroll
+				for (int kk = 0; kk < 5; kk++) nHW[i][kk] = hw[kk];
+			}
+		}
+	};
+	// slot records of block b's leading interior slots (up to KS rounds), and W0, metrics and CSR
+	// codes of its own cells
+	auto issueRec = [&](int b, int4 mb) {
+		if (b < 0) return;
+		const int s0 = uni(mb.x), nr = min(uni(mb.y) >> 16, KS * NTH);
+#pragma unroll
+		for (int r = 0; r < KS; r++) {
+			const int j = r * NTH + t;
+			if (j < nr) {
+				nPk[r] = ldStream<NT>(p.pack + s0 + j);
+				nNx[r] = ldStream<NT>(p.nx + s0 + j), nNy[r] = ldStream<NT>(p.ny + s0 + j);
+				nNz[r] = ldStream<NT>(p.nz + s0 + j), nS[r] = ldStream<NT>(p.S + s0 + j);
+			}
+		}
+		const int c0 = b * CB, nc = min(CB, m.nElem - c0);
+		const int e = c0 + t;
+		if (t < nc) {
+			if (!firstStage) {
+#pragma unroll
+				for (int kk = 0; kk < 5; kk++) nw0[kk] = ldStream<NT>(W0 + kk * n + e);
+			}
+			nvol = ldStream<NT>(m.c.vol + e);
+			nsx = ldStream<NT>(m.c.sx + e);
+			nsy = ldStream<NT>(m.c.sy + e);
+			nsz = ldStream<NT>(m.c.sz + e);
+			const uint16_t *cl = p.csr + (size_t) e * p.csrStride;
+			if (p.csrStride == 4 || p.csrStride == 6) {
+				// rows of 8 or 12 B, 4 B aligned
+				const uint32_t *cw = reinterpret_cast<const uint32_t *>(cl);
+				ncw[0] = ldStream<NT>(cw);
+				ncw[1] = ldStream<NT>(cw + 1);
+				ncw[2] = p.csrStride == 6 ? ldStream<NT>(cw + 2) : 0xFFFFFFFFu;
+			} else {
+				uint32_t h[6];
+#pragma unroll
+				for (int i = 0; i < 6; i++) h[i] = i < p.csrStride ? (uint32_t) ldStream<NT>(cl + i) : 0xFFFFu;
+#pragma unroll
+				for (int i = 0; i < 3; i++) ncw[i] = h[2 * i] | h[2 * i + 1] << 16;
+			}
+		} else {
+#pragma unroll
+			for (int i = 0; i < 3; i++) ncw[i] = 0xFFFFFFFFu;
+		}
+	};
+
+	const int4 none = make_int4(0, 0, 0, 0);
+	auto metaOf = [&](int bb) { return bb >= 0 ? p.blockMeta[bb] : none; };
+	int b = blockAt(0);
+	int4 mc = metaOf(b), mn = metaOf(blockAt(1)), mnn = metaOf(blockAt(2));
+	issueIds(b, mc);
+	issueW(b);
+	issueRec(b, mc);
+	issueIds(blockAt(1), mn);
+	for (int it = 0; b >= 0; it++) {
+		const int bn = blockAt(it + 1), bnn = blockAt(it + 2);
+		const int4 mnnn = metaOf(blockAt(it + 3)); // used three blocks on
+		const int c0 = b * CB, nc = min(CB, m.nElem - c0);
+		const int s0 = uni(mc.x), ns = uni(mc.y) & 0xFFFF, nInt = uni(mc.y) >> 16, nh = uni(mc.w);
+		// leading interior slots whose records are in registers; the rest read theirs from memory
+		const int nReg = min(nInt, KS * NTH);
+		const bool own = t < nc;
+		const int e = c0 + t;
+		// take over the prefetched block
+		double w0[5], vol = nvol, sx = nsx, sy = nsy, sz = nsz;
+#pragma unroll
+		for (int kk = 0; kk < 5; kk++) w0[kk] = nw0[kk];
+		uint32_t cw[3];
+#pragma unroll
+		for (int i = 0; i < 3; i++) cw[i] = ncw[i];
+		uint32_t cPk[KS];
+		double cNx[KS], cNy[KS], cNz[KS], cS[KS];
+#pragma unroll
+		for (int r = 0; r < KS; r++) cPk[r] = nPk[r], cNx[r] = nNx[r], cNy[r] = nNy[r], cNz[r] = nNz[r], cS[r] = nS[r];
+		// halo states -> primitives, own W -> the flux rows (turned into primitives after the barrier)
+#pragma unroll
+		for (int i = 0; i < KH; i++) {
+			const int h = i * NTH + t;
+			if (h < nh) {
+				const int l = CB + h;
+				const double r = nHW[i][0];
+				double u, v, w, pr, E, H;
+				primOf<MK>(r, nHW[i][1], nHW[i][2], nHW[i][3], nHW[i][4], q.gamma, u, v, w, pr, E, H);
+				ls[0 * cap + l] = r;
+				ls[1 * cap + l] = u;
+				ls[2 * cap + l] = v;
+				ls[3 * cap + l] = w;
+				ls[4 * cap + l] = pr;
+				ls[5 * cap + l] = H;
+				lsq[l] = sqrtN(r);
+				lcs[l] = sqrtN(divK<MK>(q.gamma * pr, r));
+			}
+		}
+#pragma unroll
+		for (int i = 0; i < kCons; i++) {
+			const int j = i * NTH + t;
+			if (2 * j < 5 * nc) {
+				lf[2 * j] = nW[i].x;
+				lf[2 * j + 1] = nW[i].y;
+			}
+		}
+		__syncthreads();
+		double Eown = 0.0;
+		if (own) {
+			const double *wt = lf + 5 * t;
+			const double r = wt[0];
+			double u, v, w, pr, E, H;
+			primOf<MK>(r, wt[1], wt[2], wt[3], wt[4], q.gamma, u, v, w, pr, E, H);
+			ls[t] = r;
+			ls[cap + t] = u;
+			ls[2 * cap + t] = v;
+			ls[3 * cap + t] = w;
+			ls[4 * cap + t] = pr;
+			ls[5 * cap + t] = H;
+			lsq[t] = sqrtN(r);
+			lcs[t] = sqrtN(divK<MK>(q.gamma * pr, r));
+			Eown = E;
+		}
+		__syncthreads();
+		issueW(bn);
+		issueIds(bnn, mnn);
+
+		// ---- face fluxes, once each, into LDS. The block's leading interior slots first, from LDS
+		// and registers only (no load in this loop, so none waits for the next block's loads);
+		// then the rest (boundary and partition interface faces), whose records and ghost states
+		// are read from memory
+		auto lprim = [&](int l) -> Prim {
+			return {ls[l], ls[cap + l], ls[2 * cap + l], ls[3 * cap + l], ls[4 * cap + l], ls[5 * cap + l]};
+		};
+#pragma unroll 1
+		for (int r = 0; r * NTH < nReg; r++) {
+			const int j = r * NTH + t;
+			const uint32_t pk = cPk[0];
+			const double nx = cNx[0], ny = cNy[0], nz = cNz[0], S = cS[0];
+#pragma unroll
+			for (int z = 0; z + 1 < KS; z++)
+				cPk[z] = cPk[z + 1], cNx[z] = cNx[z + 1], cNy[z] = cNy[z + 1], cNz[z] = cNz[z + 1], cS[z] = cS[z + 1];
+			if (j >= nReg) continue;
+			const int l0 = (int) (pk & 0x7FFu), l1 = (int) ((pk >> 11) & 0x7FFu);
+			const Prim L = lprim(l0);
+			auto emit = [&](int kk, double f) { lf[kk * ms + j] = f * S; };
+			if constexpr (SCHEME == 0) {
+				roeFluxEmit<MK>(L, lprim(l1), lsq[l0], lsq[l1], lcs[l0], nx, ny, nz, q.gamma, emit);
+			} else {
+				double F[5];
+				ausmFluxA(L, lprim(l1), lcs[l0], lcs[l1], nx, ny, nz, q.gamma, F);
+#pragma unroll
+				for (int kk = 0; kk < 5; kk++) emit(kk, F[kk]);
+			}
+		}
+		for (int j = nReg + t; j < ns; j += NTH) {
+			const int s = s0 + j;
+			const uint32_t pk = ldStream<NT>(p.pack + s);
+			const double nx = ldStream<NT>(p.nx + s), ny = ldStream<NT>(p.ny + s), nz = ldStream<NT>(p.nz + s);
+			const double S = ldStream<NT>(p.S + s);
+			const int l0 = (int) (pk & 0x7FFu), l1 = (int) ((pk >> 11) & 0x7FFu), kind = (int) (pk >> 22);
+			const Prim L = lprim(l0);
+			auto emit = [&](int kk, double f) { lf[kk * ms + j] = f * S; };
+			if (SCHEME == 0 && kind == 0) {
+				roeFluxEmit<MK>(L, lprim(l1), lsq[l0], lsq[l1], lcs[l0], nx, ny, nz, q.gamma, emit);
+				continue;
+			}
+			double F[5];
+			if (kind == 0) {
+				if constexpr (SCHEME == 1) ausmFluxA(L, lprim(l1), lcs[l0], lcs[l1], nx, ny, nz, q.gamma, F);
+				else schemeFlux<SCHEME>(L, lprim(l1), nx, ny, nz, q.gamma, F);
+			} else if (kind == 4) {
+				schemeFlux<SCHEME>(L, loadPrim(in, p.e1[s]), nx, ny, nz, q.gamma, F);
+			} else {
+				const int c = p.e1[s];
+				Ghost G;
+				if (kind == 3) {
+					const double *w5 = Win + (size_t) 5 * (c0 + l0);
+					const double Ei = quot<MK>(w5[4], w5[0], recipN(w5[0]));
+					G = farfieldGhost(in, c, machOf(L.rho, L.u, L.v, L.w, L.p, q.gamma), nx, ny, nz, q, L, Ei);
+				} else {
+					G = mirrorGhost(L.rho, L.u, L.v, L.w, L.p, nx, ny, nz, q.gamma);
+				}
+				if (kind == 3 || rmsPartial) storeGhost(out, c, G);
+				if (kind == 1) {
+					F[0] = 0;
+					F[1] = nx * G.p;
+					F[2] = ny * G.p;
+					F[3] = nz * G.p;
+					F[4] = 0;
+				} else if (kind == 2) {
+					schemeFlux<SCHEME>(L, Prim{G.rho, G.u, G.v, G.w, G.p, G.H}, nx, ny, nz, q.gamma, F);
+				} else {
+					double V = G.u * nx + G.v * ny + G.w * nz;
+					F[0] = G.rho * V;
+					F[1] = G.rho * G.u * V + G.p * nx;
+					F[2] = G.rho * G.v * V + G.p * ny;
+					F[3] = G.rho * G.w * V + G.p * nz;
+					F[4] = G.rho * G.H * V;
+				}
+			}
+#pragma unroll
+			for (int kk = 0; kk < 5; kk++) emit(kk, F[kk]);
+		}
+		issueRec(bn, mn);
+		__syncthreads();
+
+		// ---- one thread per cell: gather in reference order, local dt, RK update
+		double r2 = 0.0;
+		if (own) {
+			double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+			for (int i = 0; i < 6; i++) {
+				const unsigned cd = (cw[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
+				if (cd == 0xFFFFu) break;
+				const int j = (int) (cd & 0x7FFFu);
+				if (cd & 0x8000u) {
+#pragma unroll
+					for (int kk = 0; kk < 5; kk++) r[kk] -= lf[kk * ms + j];
+				} else {
+#pragma unroll
+					for (int kk = 0; kk < 5; kk++) r[kk] += lf[kk * ms + j];
+				}
+			}
+			const double rho = ls[t], u = ls[cap + t], v = ls[2 * cap + t], w = ls[3 * cap + t];
+			if (firstStage) {
+				w0[0] = rho;
+				w0[1] = u * rho;
+				w0[2] = v * rho;
+				w0[3] = w * rho;
+				w0[4] = Eown * rho;
+#pragma unroll
+				for (int kk = 0; kk < 5; kk++) W0[kk * n + e] = w0[kk];
+			}
+			const double c = lcs[t];
+			double lx = 0.5 * (fabs(u) + c) * sx;
+			double ly = 0.5 * (fabs(v) + c) * sy;
+			double lz = 0.5 * (fabs(w) + c) * sz;
+			double dt = divK<MK>(q.cfl * vol, lx + ly + lz);
+			const double yv = recipN(vol);
+			double *ow = Wout + (size_t) 5 * e;
+			ow[0] = w0[0] - quot<MK>(alpha * dt * r[0], vol, yv);
+			ow[1] = w0[1] - quot<MK>(alpha * dt * r[1], vol, yv);
+			ow[2] = w0[2] - quot<MK>(alpha * dt * r[2], vol, yv);
+			ow[3] = w0[3] - quot<MK>(alpha * dt * r[3], vol, yv);
+			ow[4] = w0[4] - quot<MK>(alpha * dt * r[4], vol, yv);
+			r2 = r[0] * r[0];
+		}
+		if (rmsPartial) {
+			__syncthreads();
+			const double tt = blockSumN<NTH>(r2, lds);
+			if (t == 0) rmsPartial[b] = tt;
+		}
+		__syncthreads(); // the next block overwrites LDS
+		b = bn;
+		mc = mn;
+		mn = mnn;
+		mnn = mnnn;
+	}
+}
+
 // self-check of sqrtN, recipN and divN against the library square root and division (bitwise)
 __device__ __forceinline__ uint64_t mixBits(uint64_t z) {
 	z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
@@ -1318,6 +1653,54 @@ void launchSmooth(const double *R0, double *R1, double *R2, const int *nbr, cons
 	                   nPart);
 }
 
+// The pipelined persistent stage: the shape a plan runs with (threads, halo cells and slot
+// rounds held per thread), 0 when it runs the one-block-per-workgroup kernel instead
+namespace {
+struct PipeShape {
+	int nth = 0, kh = 0;
+	bool fixed = false;
+};
+PipeShape pipeShape(const DevPlan &p) {
+	PipeShape s;
+	if (p.cellsPerBlock != 256) return s;
+	const size_t lds = stageFusedLdsBytes(256, p.maxSlots, p.maxHalo, 2);
+	if (2 * lds > 160 * 1024) {
+		if (p.maxHalo <= 2 * 512) s.nth = 512, s.kh = 2;
+	} else if (p.maxHalo <= 2 * 256) {
+		s.nth = 256, s.kh = p.maxHalo <= 256 ? 1 : 2;
+	}
+	s.fixed = stageFusedFixed(256, p.maxSlots, p.maxHalo, 2);
+	return s;
+}
+// slot records held: 3 rounds of 256 (tetrahedral blocks), 2 of 512
+template<int S, bool F, int NTH, int KH>
+const void *pipeKernel() {
+	return reinterpret_cast<const void *>(&kStagePipe<S, F, NTH, KH, NTH == 512 ? 2 : 3>);
+}
+const void *pipeKernelFor(int scheme, const PipeShape &s) {
+	if (s.nth == 512) return scheme == 1 ? (s.fixed ? pipeKernel<1, true, 512, 2>() : pipeKernel<1, false, 512, 2>())
+	                                      : (s.fixed ? pipeKernel<0, true, 512, 2>() : pipeKernel<0, false, 512, 2>());
+	if (s.kh == 1) return scheme == 1 ? (s.fixed ? pipeKernel<1, true, 256, 1>() : pipeKernel<1, false, 256, 1>())
+	                                   : (s.fixed ? pipeKernel<0, true, 256, 1>() : pipeKernel<0, false, 256, 1>());
+	return scheme == 1 ? (s.fixed ? pipeKernel<1, true, 256, 2>() : pipeKernel<1, false, 256, 2>())
+	                   : (s.fixed ? pipeKernel<0, true, 256, 2>() : pipeKernel<0, false, 256, 2>());
+}
+}// namespace
+
+int stagePipeGrid(const DevPlan &p, int scheme, int device, int xcds) {
+	const PipeShape s = pipeShape(p);
+	if (!s.nth) return 0;
+	const size_t lds = stageFusedLdsBytes(256, p.maxSlots, p.maxHalo, 2);
+	int perCU = 0, cus = 0;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, pipeKernelFor(scheme, s), s.nth, lds) != hipSuccess ||
+	    perCU <= 0)
+		return 0;
+	if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) return 0;
+	const int X = std::max(1, xcds);
+	const int grid = perCU * cus / X * X;
+	return grid >= X ? grid : 0;
+}
+
 void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState out, const double *Win, double *Wout,
                       int layout, const DevFree &q, int scheme, int integ, double *W0, double alpha, int firstStage,
                       int machDerived, double *rmsPartial, int parity, const StageTuning &t, hipStream_t st) {
@@ -1329,6 +1712,29 @@ void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState 
 	// halos) run with 512 threads, so that two waves per SIMD share it
 	const bool wide = integ != 1 && p.cellsPerBlock == 256 && 2 * lds > 160 * 1024;
 	const int x = t.xcds;
+	if (t.pipeGrid > 0 && integ != 1 && layout == 2 && t.nontemporal && t.sharedRecip) {
+		const PipeShape ps = pipeShape(p);
+		if (ps.nth) {
+			const size_t lp = stageFusedLdsBytes(256, p.maxSlots, p.maxHalo, 2);
+			const int g = t.pipeGrid;
+#define E3D_PIPE(S, F, NTH, KH)                                                                                   \
+	hipLaunchKernelGGL((kStagePipe<S, F, NTH, KH, NTH == 512 ? 2 : 3>), dim3(g), dim3(NTH), lp, st, m, p, in, out, Win, \
+	                   Wout, q, W0, alpha, firstStage, rmsPartial, x, rev)
+#define E3D_PIPE_F(S, NTH, KH)            \
+	if (ps.fixed) E3D_PIPE(S, true, NTH, KH); \
+	else E3D_PIPE(S, false, NTH, KH)
+#define E3D_PIPE_S(S)                                    \
+	if (ps.nth == 512) { E3D_PIPE_F(S, 512, 2); }         \
+	else if (ps.kh == 1) { E3D_PIPE_F(S, 256, 1); }      \
+	else { E3D_PIPE_F(S, 256, 2); }
+			if (scheme == 1) { E3D_PIPE_S(1) }
+			else { E3D_PIPE_S(0) }
+#undef E3D_PIPE_S
+#undef E3D_PIPE_F
+#undef E3D_PIPE
+			return;
+		}
+	}
 #define E3D_LAUNCH_B(S, N, K, B)                                                                               \
 	do {                                                                                                       \
 		if (integ == 1)                                                                                        \
