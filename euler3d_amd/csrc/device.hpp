@@ -117,6 +117,9 @@ struct StageTuning {
 	bool alternate = true;
 	// > 0: the persistent, software-pipelined stage with this many workgroups (stagePipeGrid)
 	int pipeGrid = 0;
+	// threads of a 256-cell block whose LDS fits one workgroup per CU, when the pipelined stage
+	// does not run it (768: 12 waves; 149 against 161 us per stage on the 1.18M-hex wing)
+	int wideThreads = 768;
 };
 
 // Fused RK stage: boundary ghosts, face fluxes (once per block, in LDS), residual gather, local

@@ -705,9 +705,9 @@ __device__ __forceinline__ int logicalBlock(int w, int nBlocks, int xcds, bool r
 // Both are odd, so the rows of one cell (and the five fluxes of one slot) sit in different banks.
 constexpr int kFixCap = 441, kFixSlots = 637;
 
-// NTH threads serve the CB cells of a block (NTH = CB, or 2 CB where a block's LDS leaves room for
-// one workgroup per CU only: more waves then share the same LDS); threads t >= CB take part in the
-// loads and the flux loop, and the gather and update run on the first CB.
+// NTH threads serve the CB cells of a block (NTH = CB, or 2 or 3 CB where a block's LDS leaves room
+// for one workgroup per CU only: more waves then share the same LDS); threads t >= CB take part in
+// the loads and the flux loop, and the gather and update run on the first CB.
 template<int SCHEME, bool NT, bool MK, int INTEG, int LAYOUT, int CB, bool FIXED = false, int NTH = CB>
 __global__ void __launch_bounds__(NTH) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out,
                                                       const double *Win, double *Wout, DevFree q, double *W0,
@@ -1709,7 +1709,7 @@ void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState 
 	const size_t lds = stageFusedLdsBytes(p.cellsPerBlock, p.maxSlots, p.maxHalo, integ == 1 ? 0 : layout);
 	const bool fixed = integ != 1 && stageFusedFixed(p.cellsPerBlock, p.maxSlots, p.maxHalo, layout);
 	// 256-cell blocks whose LDS leaves room for one workgroup per CU (hexahedral meshes with large
-	// halos) run with 512 threads, so that two waves per SIMD share it
+	// halos) run with 768 (or 512) threads, so that three (two) waves per SIMD share it
 	const bool wide = integ != 1 && p.cellsPerBlock == 256 && 2 * lds > 160 * 1024;
 	const int x = t.xcds;
 	if (t.pipeGrid > 0 && integ != 1 && layout == 2 && t.nontemporal && t.sharedRecip) {
@@ -1740,6 +1740,9 @@ void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState 
 		if (integ == 1)                                                                                        \
 			hipLaunchKernelGGL((kStageFused<S, N, K, 1, 0, B>), dim3(p.nBlocks), dim3(B), lds, st, m, p, in, out, \
 			                   Win, Wout, q, W0, alpha, 0, machDerived, rmsPartial, x, rev);                      \
+		else if (layout == 2 && wide && t.wideThreads == 768)                                                  \
+			hipLaunchKernelGGL((kStageFused<S, N, K, 0, 2, 256, false, 768>), dim3(p.nBlocks), dim3(768), lds, st, m, \
+			                   p, in, out, Win, Wout, q, W0, alpha, firstStage, machDerived, rmsPartial, x, rev); \
 		else if (layout == 2 && wide)                                                                          \
 			hipLaunchKernelGGL((kStageFused<S, N, K, 0, 2, 256, false, 512>), dim3(p.nBlocks), dim3(512), lds, st, m, \
 			                   p, in, out, Win, Wout, q, W0, alpha, firstStage, machDerived, rmsPartial, x, rev); \

@@ -5,8 +5,9 @@ The pipelined kernel runs by default only where a block's LDS leaves room for on
 compares bit for bit with the oracle). Here E3D_PIPE forces it on and off for every plan it fits,
 on tetrahedral, hexahedral and O-grid meshes, Roe and AUSM, with both XCD orders, from a seeded
 (non-uniform) state and from freestream: states and RMS sums after several RK5 iterations must be
-the same bits. The iterations cross the first stage (W0 stored), the last (ghosts and RMS
-partials stored) and the alternating block walk.
+the same bits. The hexahedral wing's plan fits one workgroup per CU, so its one-block-per-workgroup
+runs also cover the 768- and 512-thread forms of that kernel. The iterations cross the first stage
+(W0 stored), the last (ghosts and RMS partials stored) and the alternating block walk.
 """
 from __future__ import annotations
 
@@ -40,9 +41,10 @@ MESHES = {
 }
 
 
-def run(monkeypatch, mesh, prm, pipe: int, xcds: int, start=None, iters=7):
+def run(monkeypatch, mesh, prm, pipe: int, xcds: int, start=None, iters=7, wide_threads=768):
     monkeypatch.setenv("E3D_PIPE", str(pipe))
     monkeypatch.setenv("E3D_XCDS", str(xcds))
+    monkeypatch.setenv("E3D_WIDE_THREADS", str(wide_threads))
     with EulerSolver(mesh, prm) as s:
         if start is not None:
             s.set_state(start)
@@ -65,7 +67,8 @@ def test_gpu_pipelined_stage_same_bits(monkeypatch, name, scheme):
     for xcds in (8, 1):
         for st in (None, start):
             ref_sums, ref = run(monkeypatch, m, prm, 0, xcds, st)
-            sums, got = run(monkeypatch, m, prm, 1, xcds, st)
-            assert bits_equal(sums, ref_sums), (name, scheme, xcds)
-            for k in FIELDS:
-                assert bits_equal(got[k], ref[k]), (name, scheme, xcds, k)
+            for pipe, wt in ((1, 768), (0, 512)):
+                sums, got = run(monkeypatch, m, prm, pipe, xcds, st, wide_threads=wt)
+                assert bits_equal(sums, ref_sums), (name, scheme, xcds, pipe, wt)
+                for k in FIELDS:
+                    assert bits_equal(got[k], ref[k]), (name, scheme, xcds, pipe, wt, k)
