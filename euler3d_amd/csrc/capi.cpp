@@ -224,7 +224,8 @@ void runStage(e3d_ctx *c, int stage, double *rmsPartial, bool timed) {
 }
 
 const char *stageKernelName(const e3d_ctx *c) {
-	if (c->fused && !c->prm.residual_smoothing && c->prm.time_integration != 1) return "kStageFused";
+	if (c->fused && !c->prm.residual_smoothing && c->prm.time_integration != 1)
+		return c->tune.pipeGrid > 0 && c->tune.nontemporal && c->tune.sharedRecip ? "kStagePipe" : "kStageFused";
 	return c->prm.residual_smoothing ? "kBc+kFlux+kGather+kSmooth+kStage" : "kBc+kFlux+kStage";
 }
 
