@@ -95,8 +95,14 @@ struct e3d_batch {
 	std::vector<CaseHost> cs;
 	std::vector<int> statusInit; // slots whose status must be (re)written at the next activation
 	bool dirty = false;
-	// guards the host-side tables: admissions may run on other threads while iterations run
+	// guards the host-side tables: admissions may run on other threads while iterations run. No
+	// call waits for queued iterations while holding it, so an admission is never held up by them.
 	std::mutex mu;
+	// serializes the calls that read device status through the pinned hStatus (taken before mu)
+	std::mutex pollMu;
+	// recorded after the last table copies from the pinned staging (activate) and after a status
+	// copy (poll, status, wait)
+	hipEvent_t staged = nullptr, polled = nullptr;
 	// launch description and the captured iteration
 	// one launch per class of cases: cases are grouped by flux scheme and by how many
 	// of their workgroups fit on a CU (their LDS need), so that a case with large blocks does not
@@ -148,7 +154,8 @@ bool sameLaunch(const e3d::MixLaunch &a, const e3d::MixLaunch &c) {
 // Brings the device tables in line with the resident cases (after admissions or removals).
 void activate(e3d_batch *b) {
 	if (!b->dirty) return;
-	b->sync(); // the pinned staging below may still be read by earlier copies
+	// the pinned staging written below may still be read by the previous activation's copies
+	if (b->staged) hipCheck(hipEventSynchronize(b->staged), "hipEventSynchronize");
 	// classes: (scheme, workgroups of the case's own layout that fit on a CU)
 	struct Cls {
 		int scheme, perCU, maxSlots = 2, maxHalo = 0;
@@ -177,6 +184,40 @@ void activate(e3d_batch *b) {
 		k->blocks += (size_t) c.nBlocks;
 		k->slots.push_back(s);
 		total += (size_t) c.nBlocks;
+	}
+	// One launch per class keeps a large-LDS case from lowering the others' occupancy, but every
+	// launch costs about one round of workgroups. When a scheme's cases are few and small, the
+	// stage is launch-bound and one launch with the largest layout is cheaper: the classes of a
+	// scheme are merged when 1 + rounds(all blocks at the merged occupancy) is below the sum of
+	// 1 + rounds(blocks) over the classes. E3D_MIX_CLASSES=merge|split forces either. The launch
+	// shape changes no result.
+	{
+		const char *env = std::getenv("E3D_MIX_CLASSES");
+		const int force = env ? (std::string(env) == "merge" ? 1 : std::string(env) == "split" ? 0 : -1) : -1;
+		const size_t cu = (size_t) std::max(1, b->nCU);
+		auto rounds = [&](size_t blocks, int perCU) { return (blocks + cu * perCU - 1) / (cu * perCU); };
+		for (int scheme = 0; scheme < 2 && force != 0; scheme++) {
+			std::vector<size_t> idx;
+			for (size_t i = 0; i < cls.size(); i++)
+				if (cls[i].scheme == scheme) idx.push_back(i);
+			if (idx.size() < 2) continue;
+			Cls m{scheme, 1};
+			size_t sep = 0;
+			for (size_t i : idx) {
+				const Cls &k = cls[i];
+				sep += 1 + rounds(k.blocks, k.perCU);
+				m.maxSlots = std::max(m.maxSlots, k.maxSlots);
+				m.maxHalo = std::max(m.maxHalo, k.maxHalo);
+				m.blocks += k.blocks;
+				m.slots.insert(m.slots.end(), k.slots.begin(), k.slots.end());
+			}
+			m.perCU = e3d::mixResidentPerCU((m.maxSlots + 1) & ~1, e3d::kMixBlock + m.maxHalo, 1 << scheme,
+			                                b->tune.nontemporal);
+			if (force != 1 && 1 + rounds(m.blocks, m.perCU) >= sep) continue;
+			std::sort(m.slots.begin(), m.slots.end());
+			for (size_t j = idx.size(); j-- > 0;) cls.erase(cls.begin() + (long) idx[j]);
+			cls.push_back(std::move(m));
+		}
 	}
 	if (total > b->hBlocksCap) {
 		if (b->hBlocks) (void) hipHostFree(b->hBlocks);
@@ -234,6 +275,7 @@ void activate(e3d_batch *b) {
 		hipCheck(hipMemcpyAsync(b->dStatus + s, &st, sizeof(st), hipMemcpyHostToDevice, b->stream), "H2D status");
 	}
 	b->statusInit.clear();
+	hipCheck(hipEventRecord(b->staged, b->stream), "hipEventRecord");
 	b->Ls = std::move(Ls);
 	b->dirty = false;
 }
@@ -328,6 +370,8 @@ int e3d_batch_create(int device, const e3d_batch_opts *opts, e3d_batch **out) {
 		hipCheck(hipSetDevice(device), "hipSetDevice");
 		hipCheck(hipDeviceGetAttribute(&b->nCU, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
 		hipCheck(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+		hipCheck(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming), "hipEventCreate");
+		hipCheck(hipEventCreateWithFlags(&b->polled, hipEventDisableTiming), "hipEventCreate");
 		b->tune.wgPerCU = o.wg_per_cu;
 		b->tune.xcds = o.xcds;
 		b->tune.nontemporal = o.nontemporal != 0;
@@ -659,9 +703,13 @@ int e3d_batch_run(e3d_batch *b, int n) {
 int e3d_batch_wait(e3d_batch *b) {
 	if (!b) return fail(E3D_ERR_ARG, "null batch");
 	return guarded([&] {
-		std::lock_guard<std::mutex> lk(b->mu);
-		hipCheck(hipSetDevice(b->device), "hipSetDevice");
-		b->sync();
+		std::lock_guard<std::mutex> pk(b->pollMu);
+		{
+			std::lock_guard<std::mutex> lk(b->mu);
+			hipCheck(hipSetDevice(b->device), "hipSetDevice");
+			hipCheck(hipEventRecord(b->polled, b->stream), "hipEventRecord");
+		}
+		hipCheck(hipEventSynchronize(b->polled), "hipEventSynchronize");
 		return E3D_OK;
 	});
 }
@@ -669,13 +717,17 @@ int e3d_batch_wait(e3d_batch *b) {
 int e3d_batch_status(e3d_batch *b, int id, int *done, int *iters, double *rms0, double *rmsLast) {
 	if (!b || id < 0 || id >= b->nSlots) return fail(E3D_ERR_ARG, "no such case");
 	return guarded([&] {
-		std::lock_guard<std::mutex> lk(b->mu);
-		if (!b->cs[id].ready) return fail(E3D_ERR_ARG, "no such case");
-		hipCheck(hipSetDevice(b->device), "hipSetDevice");
-		activate(b);
+		std::lock_guard<std::mutex> pk(b->pollMu);
 		e3d::MixStatus *h = b->hStatus + id;
-		hipCheck(hipMemcpyAsync(h, b->dStatus + id, sizeof(*h), hipMemcpyDeviceToHost, b->stream), "D2H status");
-		b->sync();
+		{
+			std::lock_guard<std::mutex> lk(b->mu);
+			if (!b->cs[id].ready) return fail(E3D_ERR_ARG, "no such case");
+			hipCheck(hipSetDevice(b->device), "hipSetDevice");
+			activate(b);
+			hipCheck(hipMemcpyAsync(h, b->dStatus + id, sizeof(*h), hipMemcpyDeviceToHost, b->stream), "D2H status");
+			hipCheck(hipEventRecord(b->polled, b->stream), "hipEventRecord");
+		}
+		hipCheck(hipEventSynchronize(b->polled), "hipEventSynchronize");
 		if (done) *done = h->done;
 		if (iters) *iters = h->iter;
 		if (rms0) *rms0 = h->rms0;
@@ -690,13 +742,19 @@ int e3d_batch_status(e3d_batch *b, int id, int *done, int *iters, double *rms0, 
 int e3d_batch_poll(e3d_batch *b, int *ids, int *done, int *iters, double *rmsLast, int cap, int *n) {
 	if (!b || !n) return fail(E3D_ERR_ARG, "null argument");
 	return guarded([&] {
+		std::lock_guard<std::mutex> pk(b->pollMu);
+		{
+			std::lock_guard<std::mutex> lk(b->mu);
+			hipCheck(hipSetDevice(b->device), "hipSetDevice");
+			activate(b);
+			hipCheck(hipMemcpyAsync(b->hStatus, b->dStatus, sizeof(e3d::MixStatus) * b->nSlots, hipMemcpyDeviceToHost,
+			                        b->stream),
+			         "D2H status");
+			hipCheck(hipEventRecord(b->polled, b->stream), "hipEventRecord");
+		}
+		// the copy waits for every iteration queued so far; admissions go on meanwhile
+		hipCheck(hipEventSynchronize(b->polled), "hipEventSynchronize");
 		std::lock_guard<std::mutex> lk(b->mu);
-		hipCheck(hipSetDevice(b->device), "hipSetDevice");
-		activate(b);
-		hipCheck(hipMemcpyAsync(b->hStatus, b->dStatus, sizeof(e3d::MixStatus) * b->nSlots, hipMemcpyDeviceToHost,
-		                        b->stream),
-		         "D2H status");
-		b->sync();
 		int k = 0;
 		for (int s = 0; s < b->nSlots; s++) {
 			if (!b->cs[s].ready) continue;
@@ -717,6 +775,7 @@ int e3d_batch_result(e3d_batch *b, int id, double *hist, int cap, int *nHist, do
                      double cmOut[3]) {
 	if (!b || id < 0 || id >= b->nSlots) return fail(E3D_ERR_ARG, "no such case");
 	return guarded([&] {
+		std::lock_guard<std::mutex> pk(b->pollMu);
 		std::lock_guard<std::mutex> lk(b->mu);
 		if (!b->cs[id].ready) return fail(E3D_ERR_ARG, "no such case");
 		hipCheck(hipSetDevice(b->device), "hipSetDevice");
@@ -762,6 +821,7 @@ int e3d_batch_result(e3d_batch *b, int id, double *hist, int cap, int *nHist, do
 int e3d_batch_get_state(e3d_batch *b, int id, double *const fields[8]) {
 	if (!b || !fields || id < 0 || id >= b->nSlots) return fail(E3D_ERR_ARG, "no such case");
 	return guarded([&] {
+		std::lock_guard<std::mutex> pk(b->pollMu);
 		std::lock_guard<std::mutex> lk(b->mu);
 		if (!b->cs[id].ready) return fail(E3D_ERR_ARG, "no such case");
 		hipCheck(hipSetDevice(b->device), "hipSetDevice");
@@ -884,6 +944,8 @@ void e3d_batch_destroy(e3d_batch *b) {
 	(void) hipHostFree(b->hN);
 	if (b->hBlocks) (void) hipHostFree(b->hBlocks);
 	for (auto e : b->evPool) (void) hipEventDestroy(e);
+	if (b->staged) (void) hipEventDestroy(b->staged);
+	if (b->polled) (void) hipEventDestroy(b->polled);
 	(void) hipStreamDestroy(b->stream);
 	delete b;
 }

@@ -144,6 +144,27 @@ def test_graph_and_launch_shape_do_not_change_results():
         assert bits_equal(r["state"]["rho"], base["state"]["rho"]), opts
 
 
+@pytest.mark.parametrize("classes", ["merge", "split"])
+def test_merged_and_split_launch_classes_same_bits(classes, monkeypatch):
+    # a hexahedral wing (large blocks: its own launch class when split), a tetrahedral bump and an
+    # O-grid, each alone and all three in one engine with one launch per stage or one per class
+    say(f"[gpu] launch classes {classes}")
+    meshes = [Mesh.wing(ni=25, nj=13, nk=9).preprocess(1.0), Mesh.bump(8, 2, 2, seed=3).preprocess(1.0), naca()]
+    prms = [naca_params(mach=0.7, aoa=2.0), naca_params(mach=0.5, aoa=0.0), naca_params()]
+    iters = 25
+    alone = [mixed_run(m, p, iters) for m, p in zip(meshes, prms)]
+    monkeypatch.setenv("E3D_MIX_CLASSES", classes)
+    with Batch() as b:
+        ids = [b.add(m, p, drop=0.0, max_iter=iters) for m, p in zip(meshes, prms)]
+        b.run(iters)
+        for i, a in zip(ids, alone):
+            assert b.status(i)["iterations"] == iters
+            assert bits_equal(b.result(i)["rms"], a["rms"])
+            st = b.get_state(i)
+            for k in ("rho", "u", "v", "w", "p", "E", "H", "M"):
+                assert bits_equal(st[k], a["state"][k]), k
+
+
 def test_converged_case_stops_advancing():
     say("[gpu] convergence stop")
     m = naca()
