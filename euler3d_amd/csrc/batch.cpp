@@ -31,7 +31,7 @@ constexpr int kMaxClasses = 32; // (scheme, workgroups per CU) pairs
 // sizes of one case's device arrays, in the order they are carved from its allocation
 struct CaseLayout {
 	size_t nB, nH, nS, nE, nG, stride, hist, nW, nDefect = 0;
-	size_t off[24];
+	size_t off[25];
 	size_t total = 0;
 	void build() {
 		const size_t sz[] = {
@@ -45,7 +45,8 @@ struct CaseLayout {
 		    std::max<size_t>(nW, 1) * 4, std::max<size_t>(nW, 1) * 8 * 6,             // wallCell wN wC
 		    8 * 8,                                                                    // scalars
 		    std::max<size_t>(nDefect, 1) * 12,                                        // defect
-		    std::max<size_t>(nB, 1) * 4};                                             // blockInt
+		    std::max<size_t>(nB, 1) * 4,                                              // blockInt
+		    5 * nE * 4};                                                              // dW0
 		total = 0;
 		for (size_t i = 0; i < sizeof(sz) / sizeof(sz[0]); i++) {
 			off[i] = total;
@@ -551,11 +552,13 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 		// freestream start (FlowField::Initialize): W and the FP32 records (zero perturbation,
 		// stored Mach number M_inf)
 		const e3d::DevFree &q = c.fr;
+		const double wInf[5] = {q.rho_inf, q.rho_inf * q.u_inf, q.rho_inf * q.v_inf, q.rho_inf * q.w_inf, q.rho_inf * q.E_inf};
 		{
 			std::vector<double> W(5 * nE);
-			const double w[5] = {q.rho_inf, q.rho_inf * q.u_inf, q.rho_inf * q.v_inf, q.rho_inf * q.w_inf, q.rho_inf * q.E_inf};
-			for (int k = 0; k < 5; k++) std::fill(W.begin() + k * nE, W.begin() + (k + 1) * nE, w[k]);
+			for (int k = 0; k < 5; k++) std::fill(W.begin() + k * nE, W.begin() + (k + 1) * nE, wInf[k]);
 			up(14, W.data(), W.size() * 8);
+			const std::vector<float> zero(5 * nE, 0.0f);
+			up(24, zero.data(), zero.size() * 4);
 			std::vector<float> rec((nE + nG) * e3d::kRecWidth, 0.0f);
 			for (size_t i = 0; i < nE + nG; i++) rec[i * e3d::kRecWidth + e3d::R_M] = (float) q.M_inf;
 			up(15, rec.data(), rec.size() * 4);
@@ -613,6 +616,8 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 		d.sz = (const float *) at(13);
 		d.defect = defect.empty() ? nullptr : (const float *) at(22);
 		d.W = (double *) at(14);
+		d.dW0 = (float *) at(24);
+		for (int k = 0; k < 5; k++) d.consInf[k] = wInf[k];
 		d.rec[0] = (float *) at(15);
 		d.rec[1] = (float *) at(16);
 		d.partial = (double *) at(17);

@@ -686,6 +686,7 @@ int e3d_create(const e3d_mesh *m, const e3d_params *p, int device, e3d_ctx **out
 		if (const char *v = std::getenv("E3D_SHARED_RECIP")) c->tune.sharedRecip = std::atoi(v) != 0;
 		if (const char *v = std::getenv("E3D_ALTERNATE")) c->tune.alternate = std::atoi(v) != 0;
 		if (const char *v = std::getenv("E3D_WIDE_THREADS")) c->tune.wideThreads = std::atoi(v) == 512 ? 512 : 768;
+		if (const char *v = std::getenv("E3D_FIXED_SHAPE")) c->tune.fixedShape = std::max(-1, std::min(2, std::atoi(v)));
 		// blocks of 256 cells, or of 128 where a 256-cell block does not fit in a CU's LDS;
 		// E3D_BLOCK_CELLS=128|256 forces one. (Smaller blocks keep more waves resident on
 		// hexahedral meshes with large halos, but the extra halo cells and duplicated faces cost

@@ -120,6 +120,11 @@ struct StageTuning {
 	// threads of a 256-cell block whose LDS fits one workgroup per CU, when the pipelined stage
 	// does not run it (768: 12 waves; 149 against 161 us per stage on the 1.18M-hex wing)
 	int wideThreads = 768;
+	// 256-cell blocks at the fixed LDS strides: 0 the update's inputs loaded with the block's state,
+	// 256 threads; 1 loaded after the flux loop, 320 threads at 128 registers (15 waves per CU);
+	// 2 loaded after the flux loop, 256 threads; -1 by scheme: AUSM 1 (6.3M-tet bump 345 -> 324 us
+	// per stage), Roe 2 (380 -> 377 us; 320 threads took 454 us)
+	int fixedShape = -1;
 };
 
 // Fused RK stage: boundary ghosts, face fluxes (once per block, in LDS), residual gather, local

@@ -707,9 +707,12 @@ constexpr int kFixCap = 441, kFixSlots = 637;
 
 // NTH threads serve the CB cells of a block (NTH = CB, or 2 or 3 CB where a block's LDS leaves room
 // for one workgroup per CU only: more waves then share the same LDS); threads t >= CB take part in
-// the loads and the flux loop, and the gather and update run on the first CB.
-template<int SCHEME, bool NT, bool MK, int INTEG, int LAYOUT, int CB, bool FIXED = false, int NTH = CB>
-__global__ void __launch_bounds__(NTH) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out,
+// the loads and the flux loop, and the gather and update run on the first CB. With LATE the
+// update's own inputs (W0, metrics, CSR codes) are loaded after the flux loop instead of with the
+// block's state, so that they hold no registers during it: with NTH = 5 CB / 4 the kernel then
+// fits 128 registers, and three workgroups per CU run 15 waves instead of 12.
+template<int SCHEME, bool NT, bool MK, int INTEG, int LAYOUT, int CB, bool FIXED = false, int NTH = CB, bool LATE = false>
+__global__ void __launch_bounds__(NTH) __attribute__((amdgpu_waves_per_eu(LATE ? 4 : 1))) kStageFused(DevMesh m, DevPlan p, DevState in, DevState out,
                                                       const double *Win, double *Wout, DevFree q, double *W0,
                                                       double alpha, int firstStage, int machDerived,
                                                       double *rmsPartial, int xcds, int reverse) {
@@ -766,7 +769,8 @@ __global__ void __launch_bounds__(NTH) kStageFused(DevMesh m, DevPlan p, DevStat
 	}
 	double w0[5], vol = 0, sx = 0, sy = 0, sz = 0;
 	uint16_t code[6];
-	if (own) {
+	auto loadUpdateInputs = [&]() {
+		if (!own) return;
 		if (INTEG == 1 || !firstStage) {
 #pragma unroll
 			for (int k = 0; k < 5; k++) w0[k] = ldStream<NT>(W0 + k * n + e);
@@ -786,7 +790,8 @@ __global__ void __launch_bounds__(NTH) kStageFused(DevMesh m, DevPlan p, DevStat
 #pragma unroll
 			for (int i = 0; i < 6; i++) code[i] = i < p.csrStride ? ldStream<NT>(cl + i) : (uint16_t) 0xFFFF;
 		}
-	}
+	};
+	if constexpr (!LATE) loadUpdateInputs();
 	for (int h = t; h < nh; h += NTH) {
 		const int l = CB + h;
 		if constexpr (LAYOUT < 2) {
@@ -935,6 +940,7 @@ __global__ void __launch_bounds__(NTH) kStageFused(DevMesh m, DevPlan p, DevStat
 #pragma unroll
 		for (int k = 0; k < 5; k++) emit(k, F[k]);
 	}
+	if constexpr (LATE) loadUpdateInputs();
 	__syncthreads();
 
 	// ---- one thread per cell: gather in reference order, local dt, RK update
@@ -1712,6 +1718,7 @@ void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState 
 	// halos) run with 768 (or 512) threads, so that three (two) waves per SIMD share it
 	const bool wide = integ != 1 && p.cellsPerBlock == 256 && 2 * lds > 160 * 1024;
 	const int x = t.xcds;
+	const int fixedShape = t.fixedShape >= 0 ? t.fixedShape : scheme == 1 ? 1 : 2;
 	if (t.pipeGrid > 0 && integ != 1 && layout == 2 && t.nontemporal && t.sharedRecip) {
 		const PipeShape ps = pipeShape(p);
 		if (ps.nth) {
@@ -1746,6 +1753,12 @@ void launchStageFused(const DevMesh &m, const DevPlan &p, DevState in, DevState 
 		else if (layout == 2 && wide)                                                                          \
 			hipLaunchKernelGGL((kStageFused<S, N, K, 0, 2, 256, false, 512>), dim3(p.nBlocks), dim3(512), lds, st, m, \
 			                   p, in, out, Win, Wout, q, W0, alpha, firstStage, machDerived, rmsPartial, x, rev); \
+		else if (layout == 2 && fixed && fixedShape == 1)                                                    \
+			hipLaunchKernelGGL((kStageFused<S, N, K, 0, 2, 256, true, 320, true>), dim3(p.nBlocks), dim3(320), lds, st, \
+			                   m, p, in, out, Win, Wout, q, W0, alpha, firstStage, machDerived, rmsPartial, x, rev); \
+		else if (layout == 2 && fixed && fixedShape == 2)                                                    \
+			hipLaunchKernelGGL((kStageFused<S, N, K, 0, 2, 256, true, 256, true>), dim3(p.nBlocks), dim3(256), lds, st, \
+			                   m, p, in, out, Win, Wout, q, W0, alpha, firstStage, machDerived, rmsPartial, x, rev); \
 		else if (layout == 2 && fixed)                                                                         \
 			hipLaunchKernelGGL((kStageFused<S, N, K, 0, 2, 256, true>), dim3(p.nBlocks), dim3(256), lds, st, m, p, \
 			                   in, out, Win, Wout, q, W0, alpha, firstStage, machDerived, rmsPartial, x, rev);     \

@@ -435,9 +435,16 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 					r[k] = tt;
 				}
 			}
+			// the iteration's start state: FP64 in the last stage, which writes the new W; the
+			// stages before it only form their FP32 records from the result
 			double w0[5];
+			if (stage == 4) {
 #pragma unroll
-			for (int k = 0; k < 5; k++) w0[k] = ldS<NT>(C.W + k * n + e);
+				for (int k = 0; k < 5; k++) w0[k] = ldS<NT>(C.W + k * n + e);
+			} else {
+#pragma unroll
+				for (int k = 0; k < 5; k++) w0[k] = C.consInf[k] + (double) ldS<NT>(C.dW0 + k * n + e);
+			}
 			const float vol = ldS<NT>(C.vol + e), sx = ldS<NT>(C.sx + e), sy = ldS<NT>(C.sy + e), sz = ldS<NT>(C.sz + e);
 			if (C.defect) {
 				// F_inf through the cell's unclosed remainder (exact freestream flux, FP64)
@@ -473,7 +480,10 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 			storeRec(out, e, d, M);
 			if (stage == 4) {
 #pragma unroll
-				for (int k = 0; k < 5; k++) C.W[k * n + e] = nw[k];
+				for (int k = 0; k < 5; k++) {
+					C.W[k * n + e] = nw[k];
+					C.dW0[k * n + e] = (float) (nw[k] - C.consInf[k]);
+				}
 			}
 			r0 = r[0];
 		}

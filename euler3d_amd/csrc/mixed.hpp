@@ -15,7 +15,12 @@
 // below the first residual; the stall test below retires a case there so that the FP64 path can
 // finish it.
 //
-// Cases. Every resident case owns its arrays (plan, metrics, W, two FP32 record buffers,
+// State. The FP64 conservative state W is read and written once per iteration, by the last RK
+// stage. The stages before it only need their state for the FP32 records that the fluxes read,
+// so they form it from an FP32 copy of the iteration's start state, dW0 = W - W_inf (the
+// perturbation form again), instead of reading W.
+//
+// Cases. Every resident case owns its arrays (plan, metrics, W, dW0, two FP32 record buffers,
 // per-block partial sums, residual history). A device table of MixCase descriptors and a table
 // of (case, block) pairs let one launch walk the blocks of all cases; a case's
 // arithmetic does not depend on which other cases are resident or where its blocks sit in the
@@ -59,6 +64,7 @@ struct MixCase {
 	const float *defect;
 	// state
 	double *W;      // 5 x nElem conservative state (rho, rho u, rho v, rho w, rho E), FP64
+	float *dW0;     // 5 x nElem FP32 W - W_inf of the iteration's start state (stages 1 to 4)
 	float *rec[2];  // (nElem + nGhost) x kRecWidth FP32 records, ping-pong
 	double *partial; // nBlocks partial sums of R_rho^2 (last stage)
 	double *hist;    // residual RMS per iteration, histCap entries
@@ -66,6 +72,7 @@ struct MixCase {
 	float gamma, cfl, rInf, uInf, vInf, wInf, pInf, hInf, mInf;
 	float pad1;
 	double gammaD, rInfD, uInfD, vInfD, wInfD, pInfD, hInfD;
+	double consInf[5]; // W_inf (rho, rho u, rho v, rho w, rho E) as W is initialised
 };
 
 // device-written progress of a case

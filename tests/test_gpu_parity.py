@@ -191,7 +191,8 @@ def test_gpu_reads_between_iterations_change_nothing(scheme, tmp_path):
 def test_gpu_launch_switches_do_not_change_results(scheme, monkeypatch):
     """Each launch-shape and arithmetic-sequence switch of the fused stage (library IEEE divisions
     instead of the shared-reciprocal and short sequences, cached loads, plain dispatch order, one
-    walk direction, Morton cell order, 128-cell blocks) gives the state bits of the default."""
+    walk direction, Morton cell order, 128-cell blocks, each thread shape of the fixed-stride
+    stage) gives the state bits of the default."""
     say(f"[gpu] launch switches scheme={scheme}")
     m = Mesh.bump(nx=24, ny=8, nz=8, jitter=0.1, seed=0, permute=True).preprocess(1.0)
     pd = golden_params(load_golden("bump_small"))
@@ -203,12 +204,13 @@ def test_gpu_launch_switches_do_not_change_results(scheme, monkeypatch):
 
     base_sums, base = run()
     for var, val in (("E3D_SHARED_RECIP", "0"), ("E3D_NONTEMPORAL", "0"), ("E3D_XCDS", "1"), ("E3D_ALTERNATE", "0"),
-                     ("E3D_CELL_ORDER", "morton"), ("E3D_BLOCK_CELLS", "128")):
+                     ("E3D_CELL_ORDER", "morton"), ("E3D_BLOCK_CELLS", "128"), ("E3D_FIXED_SHAPE", "0"),
+                     ("E3D_FIXED_SHAPE", "1"), ("E3D_FIXED_SHAPE", "2")):
         with monkeypatch.context() as mp:
             mp.setenv(var, val)
             sums, st = run()
         for k in FIELDS:
-            assert bits_equal(st[k], base[k]), (var, k)
+            assert bits_equal(st[k], base[k]), (var, val, k)
         # the RMS sums block partials, which follow the cell order and block size
         np.testing.assert_allclose(sums, base_sums, rtol=1e-12, err_msg=var)
 
