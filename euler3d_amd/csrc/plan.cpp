@@ -16,6 +16,7 @@
 #include <functional>
 #include <thread>
 #include <cmath>
+#include <cstdlib>
 #include <numeric>
 #include <type_traits>
 #include <stdexcept>
@@ -23,6 +24,12 @@
 namespace e3d {
 
 namespace {
+
+// E3D_HALO_GROUPING=0 keeps the Morton order inside blocks
+bool haloGrouping() {
+	const char *v = std::getenv("E3D_HALO_GROUPING");
+	return !(v && std::atoi(v) == 0);
+}
 
 int planThreads() {
 	static const int n = [] {
@@ -309,6 +316,40 @@ std::vector<int> bisectOrder(const SolverMesh &sm, int cellsPerBlock) {
 	return ids;
 }
 
+// Within each block, the cells that no other block reads come first, then the cells that other
+// blocks read as halo, grouped by the lowest-numbered such block (in their previous order inside
+// each group). A block's halo then arrives in runs of consecutive cells of each neighbouring block,
+// so that its halo loads touch fewer lines. Only the order inside blocks changes.
+void groupHaloSources(const SolverMesh &sm, int cellsPerBlock, std::vector<int> &new2old) {
+	const int n = sm.nElem;
+	std::vector<int> old2new(n);
+	for (int i = 0; i < n; i++) old2new[new2old[i]] = i;
+	const int nb = (n + cellsPerBlock - 1) / cellsPerBlock;
+	parallelFor((size_t) nb, 16, [&](size_t ba, size_t bb) {
+		std::vector<std::pair<int, int>> key;
+		for (size_t blk = ba; blk < bb; blk++) {
+			const int c0 = (int) blk * cellsPerBlock, c1 = std::min(n, c0 + cellsPerBlock);
+			key.clear();
+			for (int i = c0; i < c1; i++) {
+				const int c = new2old[i];
+				int sig = -1;
+				for (int q = sm.element2faceStart[c]; q < sm.element2faceStart[c + 1]; q++) {
+					const int f = sm.element2face[q];
+					const int a = sm.face2element[2 * (size_t) f], o = a == c ? sm.face2element[2 * (size_t) f + 1] : a;
+					if (o >= n) continue;
+					const int ob = old2new[o] / cellsPerBlock;
+					if (ob != (int) blk && (sig < 0 || ob < sig)) sig = ob;
+				}
+				key.emplace_back(sig, i);
+			}
+			std::sort(key.begin(), key.end());
+			std::vector<int> cells(key.size());
+			for (size_t k = 0; k < key.size(); k++) cells[k] = new2old[key[k].second];
+			std::copy(cells.begin(), cells.end(), new2old.begin() + c0);
+		}
+	});
+}
+
 int8_t slotKind(const SolverMesh &sm, int f, bool exchange) {
 	if (f < sm.nFaceInt) return 0;
 	switch (sm.ghostKind[f - sm.nFaceInt]) {
@@ -334,6 +375,7 @@ HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock, bool exch
 		P.new2old = mortonOrder(sm);
 	} else if (order == ORDER_BISECT) {
 		P.new2old = bisectOrder(sm, cellsPerBlock);
+		if (haloGrouping()) groupHaloSources(sm, cellsPerBlock, P.new2old);
 	} else {
 		P.new2old.resize(nE);
 		std::iota(P.new2old.begin(), P.new2old.end(), 0);
