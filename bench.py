@@ -13,7 +13,9 @@ by the mean duration of the stage kernel(s), measured with HIP events on the sol
 during the timed steps, against 8 TB/s. cpu_baseline: the CPU oracle (single-thread C
 restatement of the reference's arithmetic) timed on one RK5 iteration of the same mesh, with its
 recorded speed ratio to the compiled reference (oracle/cpu_anchor.json) and its timings on the
-NACA and wing meshes. mixed_precision: the same workload on the FP32-flux / FP64-update engine.
+NACA and wing meshes. other_scheme: the same FP64 workload with AUSM (the bump case runs both
+schemes), with its own roofline. mixed_precision: the same workload on the FP32-flux / FP64-update
+engine.
 served: a bounded Poisson stream of six requests, served by continuous batching against the plain
 FP64 path one request at a time.
 """
@@ -93,6 +95,28 @@ def mixed_precision_leg(mesh, prm, n_cells: int, n_faces: int, steps: int, warmu
                              "traffic": measured_traffic("kMixStage", n_cells, prm.scheme),
                              "bytes_per_cell_stage": bpc, "launches": launches}})
     return out
+
+
+def other_scheme_leg(mesh, pd: dict, scheme: int, n_cells: int, n_faces: int, steps: int, warmup: int) -> dict:
+    """The same FP64 workload with the other flux scheme (the 6.3M-tet bump runs both Roe and AUSM):
+    wall-clock throughput and the stage kernel's roofline, timed as the main leg."""
+    from euler3d_amd import EulerSolver, default_params
+
+    prm = default_params(**{**pd, "scheme": scheme}, orient_cl=(1, 1), orient_cd=(0, 1), orient_cm=(2, 1))
+    with EulerSolver(mesh, prm, device=0) as s:
+        s.iterate(warmup)
+        s.set_kernel_timing(True)
+        t1 = time.perf_counter()
+        s.iterate(steps)
+        wall = time.perf_counter() - t1
+        kern_ms, launches, kname = s.kernel_timing()
+    bpc = algorithmic_bytes_per_cell_stage(n_cells, n_faces)
+    achieved = bpc * n_cells / (kern_ms * 1e-3) / 1e9
+    return {"scheme": "AUSM" if scheme else "Roe", "value": n_cells * 5 * steps / wall, "unit": "cell-stage/s",
+            "ms_per_step": wall * 1e3 / steps, "dtype": "f64",
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname, n_cells, scheme),
+                         "kernel": kname, "bytes_per_cell_stage": bpc, "kernel_ms": kern_ms, "launches": launches}}
 
 
 def cpu_model() -> str:
@@ -414,6 +438,8 @@ def main() -> None:
     ap.add_argument("--scheme", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-other-scheme", action="store_true",
+                    help="skip the FP64 leg with the other flux scheme (AUSM when --scheme 0)")
     ap.add_argument("--no-serve", action="store_true", help="skip the bounded served leg")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run one plain device-to-device copy of this many MiB (a known-size kernel "
@@ -504,6 +530,11 @@ def main() -> None:
         "cpu_baseline": None,
     }
     solver.close()
+    if not a.no_other_scheme:
+        out["other_scheme"] = other_scheme_leg(mesh, pd, 1 - a.scheme, n_cells, n_faces, a.steps, a.warmup)
+        log(f"{out['other_scheme']['scheme']}: {out['other_scheme']['value']:.3e} cell-stage/s, "
+            f"{out['other_scheme']['roofline']['kernel']} {out['other_scheme']['roofline']['kernel_ms'] * 1e3:.1f} us, "
+            f"{out['other_scheme']['roofline']['frac']:.3f} of HBM peak")
     if not a.no_mixed:
         out["mixed_precision"] = mixed_precision_leg(mesh, prm, n_cells, n_faces, a.steps, a.warmup)
         log(f"mixed precision: {out['mixed_precision']['value']:.3e} cell-stage/s, kMixStage "

@@ -38,7 +38,7 @@ struct CaseLayout {
 		    (nB + 1) * 4, (nB + 1) * 4, std::max<size_t>(nH, 1) * 4, nS * 4, nS * 4,  // blockSlot blockHalo haloCell ghostOf pack
 		    nS * 4, nS * 4, nS * 4, nS * 4,                                           // nx ny nz S
 		    nE * stride * 2,                                                          // csr
-		    nE * 4, nE * 4, nE * 4, nE * 4,                                           // vol sx sy sz
+		    0, nE * 4, nE * 4, nE * 4,                                                // (unused) sx sy sz
 		    5 * nE * 8,                                                               // W
 		    (nE + nG) * e3d::kRecWidth * 4, (nE + nG) * e3d::kRecWidth * 4,           // rec[2]
 		    std::max<size_t>(nB, 1) * 8, std::max<size_t>(hist, 1) * 8,               // partial hist
@@ -545,7 +545,6 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 		upF(7, P.snz);
 		upF(8, P.sS);
 		up(9, P.csr.data(), P.csr.size() * 2);
-		upF(10, P.vol);
 		upF(11, P.sx);
 		upF(12, P.sy);
 		upF(13, P.sz);
@@ -610,7 +609,6 @@ int e3d_batch_add(e3d_batch *b, const e3d_mesh *m, const e3d_params *p, const e3
 		d.nz = (const float *) at(7);
 		d.S = (const float *) at(8);
 		d.csr = (const uint16_t *) at(9);
-		d.vol = (const float *) at(10);
 		d.sx = (const float *) at(11);
 		d.sy = (const float *) at(12);
 		d.sz = (const float *) at(13);

@@ -428,7 +428,12 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 				}
 #pragma unroll
 				for (int k = 0; k < 5; k++) {
-					const double xd = neg ? -(double) x[k] : (double) x[k];
+					const double xd = (double) (neg ? -x[k] : x[k]);
+					if (i == 0) {
+						// the first term starts the sum exactly, with no compensation
+						r[k] = xd;
+						continue;
+					}
 					const double y = xd - cmp[k];
 					const double tt = r[k] + y;
 					cmp[k] = (tt - r[k]) - y;
@@ -445,7 +450,7 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 #pragma unroll
 				for (int k = 0; k < 5; k++) w0[k] = C.consInf[k] + (double) ldS<NT>(C.dW0 + k * n + e);
 			}
-			const float vol = ldS<NT>(C.vol + e), sx = ldS<NT>(C.sx + e), sy = ldS<NT>(C.sy + e), sz = ldS<NT>(C.sz + e);
+			const float sx = ldS<NT>(C.sx + e), sy = ldS<NT>(C.sy + e), sz = ldS<NT>(C.sz + e);
 			if (C.defect) {
 				// F_inf through the cell's unclosed remainder (exact freestream flux, FP64)
 				const double cx = C.defect[3 * (size_t) e], cy = C.defect[3 * (size_t) e + 1], cz = C.defect[3 * (size_t) e + 2];
@@ -461,13 +466,16 @@ __global__ void __launch_bounds__(kB) kMixStage(MixLaunch L, int stage, double a
 			const float rho = f.r + ra.x, u = f.u + ra.y, v = f.v + ra.z, w = f.w + ra.w, p = f.p + rb.x;
 			const float c = fsqrt(f.g * p * frcp(rho));
 			const float lam = 0.5f * (fabsf(u) + c) * sx + 0.5f * (fabsf(v) + c) * sy + 0.5f * (fabsf(w) + c) * sz;
-			const double dt = (double) (C.cfl * vol * frcp(lam));
-			// W_k = W0 - ((alpha dt) R) / V (EulerSolver.cpp:441-449), FP64
-			const double ad = alpha * dt / (double) vol;
+			// W_k = W0 - ((alpha dt) R) / V (EulerSolver.cpp:441-449) with dt = CFL V / lambda: the
+			// volume cancels, so the factor is alpha CFL / lambda (FP32 to FP64), and the update FP64
+			const double ad = alpha * (double) (C.cfl * frcp(lam));
 			double nw[5];
 #pragma unroll
 			for (int k = 0; k < 5; k++) nw[k] = w0[k] - ad * r[k];
-			const double ir = 1.0 / nw[0];
+			// 1 / rho for the FP32 record only: the hardware reciprocal and one Newton step (about
+			// 2^-50 relative), not the IEEE division
+			double ir = __builtin_amdgcn_rcp(nw[0]);
+			ir = fma(ir, fma(-nw[0], ir, 1.0), ir);
 			const double nu = nw[1] * ir, nv = nw[2] * ir, nwv = nw[3] * ir, nE = nw[4] * ir;
 			const double gd = C.gammaD;
 			const double q2 = nu * nu + nv * nv + nwv * nwv;

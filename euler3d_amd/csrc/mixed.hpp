@@ -57,8 +57,9 @@ struct MixCase {
 	const uint32_t *pack;
 	const float *nx, *ny, *nz, *S;
 	const uint16_t *csr;
-	// cell metrics (FP32) and volume
-	const float *vol, *sx, *sy, *sz;
+	// cell metrics (FP32): the sums of |N_x|, |N_y|, |N_z| over the faces (the volume cancels
+	// from the update, W -= alpha CFL R / lambda)
+	const float *sx, *sy, *sz;
 	// closure defect sum(+-n S) of each cell (3 floats), null when every cell is closed: the
 	// perturbation form adds F_inf . defect back so that unclosed cells keep the reference's residual
 	const float *defect;
