@@ -27,6 +27,7 @@ namespace {
 size_t align256(size_t x) { return (x + 255) & ~(size_t) 255; }
 
 constexpr int kMaxClasses = 32; // (scheme, workgroups per CU) pairs
+constexpr int kGraphIters = 8;  // iterations of the longer captured graph
 
 // sizes of one case's device arrays, in the order they are carved from its allocation
 struct CaseLayout {
@@ -109,8 +110,10 @@ struct e3d_batch {
 	// of their workgroups fit on a CU (their LDS need), so that a case with large blocks does not
 	// lower the occupancy of every other case
 	std::vector<e3d::MixLaunch> Ls;
-	hipGraph_t graph = nullptr;
-	hipGraphExec_t exec = nullptr;
+	// one iteration, and kGraphIters iterations in a row (one graph launch instead of several:
+	// each launch costs about 8 us between graphs, a sixth of a small case's iteration)
+	hipGraph_t graph = nullptr, graphK = nullptr;
+	hipGraphExec_t exec = nullptr, execK = nullptr;
 	std::vector<e3d::MixLaunch> graphLs;
 	bool haveGraph = false;
 	// timing
@@ -126,8 +129,10 @@ namespace {
 void freeGraph(e3d_batch *b) {
 	if (b->exec) (void) hipGraphExecDestroy(b->exec);
 	if (b->graph) (void) hipGraphDestroy(b->graph);
-	b->exec = nullptr;
-	b->graph = nullptr;
+	if (b->execK) (void) hipGraphExecDestroy(b->execK);
+	if (b->graphK) (void) hipGraphDestroy(b->graphK);
+	b->exec = b->execK = nullptr;
+	b->graph = b->graphK = nullptr;
 	b->haveGraph = false;
 }
 
@@ -143,6 +148,10 @@ void capture(e3d_batch *b) {
 	enqueueIteration(b);
 	hipCheck(hipStreamEndCapture(b->stream, &b->graph), "hipStreamEndCapture");
 	hipCheck(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+	hipCheck(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+	for (int i = 0; i < kGraphIters; i++) enqueueIteration(b);
+	hipCheck(hipStreamEndCapture(b->stream, &b->graphK), "hipStreamEndCapture");
+	hipCheck(hipGraphInstantiate(&b->execK, b->graphK, nullptr, nullptr, 0), "hipGraphInstantiate");
 	b->graphLs = b->Ls;
 	b->haveGraph = true;
 }
@@ -682,7 +691,11 @@ int e3d_batch_run(e3d_batch *b, int n) {
 			}
 			return b->evPool[b->evUsed++];
 		};
-		for (int i = 0; i < n; i++) {
+		int i = 0;
+		// whole groups of iterations as one graph launch (not when each iteration is timed)
+		if (b->useGraph && !b->timing)
+			for (; i + kGraphIters <= n; i += kGraphIters) hipCheck(hipGraphLaunch(b->execK, b->stream), "hipGraphLaunch");
+		for (; i < n; i++) {
 			if (b->useGraph) {
 				hipEvent_t a = b->timing ? ev() : nullptr, z = b->timing ? ev() : nullptr;
 				if (a) hipCheck(hipEventRecord(a, b->stream), "hipEventRecord");
