@@ -590,7 +590,12 @@ int e3d_mesh_get_i32(const e3d_mesh *m, const char *name, int *out, long long ca
 	const std::vector<int> *v = nullptr;
 	std::string s(name);
 	const auto &sm = m->sm;
-	if (s == "face2element") v = &sm.face2element;
+	std::vector<int> order;
+	if (s == "deviceOrder") {
+		// bisection blocks of the default size, as e3d_create plans them
+		order = e3d::cellOrder(sm, e3d::ORDER_BISECT, e3d::kCellsPerBlock);
+		v = &order;
+	} else if (s == "face2element") v = &sm.face2element;
 	else if (s == "face2nodeStart") v = &sm.face2nodeStart;
 	else if (s == "face2node") v = &sm.face2node;
 	else if (s == "element2faceStart") v = &sm.element2faceStart;

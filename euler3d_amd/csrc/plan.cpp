@@ -15,6 +15,7 @@
 #include <atomic>
 #include <functional>
 #include <thread>
+#include <tuple>
 #include <cmath>
 #include <cstdlib>
 #include <numeric>
@@ -319,14 +320,22 @@ std::vector<int> bisectOrder(const SolverMesh &sm, int cellsPerBlock) {
 // Within each block, the cells that no other block reads come first, then the cells that other
 // blocks read as halo, grouped by the lowest-numbered such block (in their previous order inside
 // each group). A block's halo then arrives in runs of consecutive cells of each neighbouring block,
-// so that its halo loads touch fewer lines. Only the order inside blocks changes.
+// so that its halo loads touch fewer lines. Only the order inside blocks changes, and inside the
+// part of a block that each partition of a composite mesh holds: a block may straddle two
+// partitions, whose cells must keep their contiguous device ranges (the per-partition smoothing
+// decisions and coefficient sums read those ranges).
 void groupHaloSources(const SolverMesh &sm, int cellsPerBlock, std::vector<int> &new2old) {
 	const int n = sm.nElem;
 	std::vector<int> old2new(n);
 	for (int i = 0; i < n; i++) old2new[new2old[i]] = i;
 	const int nb = (n + cellsPerBlock - 1) / cellsPerBlock;
+	// partition of each device position (the bisection keeps partitions in their ranges)
+	auto partOf = [&](int i) -> int {
+		if (sm.partStart.size() <= 2) return 0;
+		return (int) (std::upper_bound(sm.partStart.begin(), sm.partStart.end(), i) - sm.partStart.begin()) - 1;
+	};
 	parallelFor((size_t) nb, 16, [&](size_t ba, size_t bb) {
-		std::vector<std::pair<int, int>> key;
+		std::vector<std::tuple<int, int, int>> key;
 		for (size_t blk = ba; blk < bb; blk++) {
 			const int c0 = (int) blk * cellsPerBlock, c1 = std::min(n, c0 + cellsPerBlock);
 			key.clear();
@@ -340,11 +349,11 @@ void groupHaloSources(const SolverMesh &sm, int cellsPerBlock, std::vector<int> 
 					const int ob = old2new[o] / cellsPerBlock;
 					if (ob != (int) blk && (sig < 0 || ob < sig)) sig = ob;
 				}
-				key.emplace_back(sig, i);
+				key.emplace_back(partOf(i), sig, i);
 			}
 			std::sort(key.begin(), key.end());
 			std::vector<int> cells(key.size());
-			for (size_t k = 0; k < key.size(); k++) cells[k] = new2old[key[k].second];
+			for (size_t k = 0; k < key.size(); k++) cells[k] = new2old[std::get<2>(key[k])];
 			std::copy(cells.begin(), cells.end(), new2old.begin() + c0);
 		}
 	});
@@ -364,6 +373,20 @@ int8_t slotKind(const SolverMesh &sm, int f, bool exchange) {
 
 }// namespace
 
+std::vector<int> cellOrder(const SolverMesh &sm, int order, int cellsPerBlock) {
+	std::vector<int> new2old;
+	if (order == ORDER_MORTON) {
+		new2old = mortonOrder(sm);
+	} else if (order == ORDER_BISECT) {
+		new2old = bisectOrder(sm, cellsPerBlock);
+		if (haloGrouping()) groupHaloSources(sm, cellsPerBlock, new2old);
+	} else {
+		new2old.resize(sm.nElem);
+		std::iota(new2old.begin(), new2old.end(), 0);
+	}
+	return new2old;
+}
+
 HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock, bool exchange) {
 	HostPlan P;
 	const int nE = sm.nElem, nF = sm.nFace;
@@ -371,15 +394,7 @@ HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock, bool exch
 	P.nFace = nF;
 	P.nFaceInt = sm.nFaceInt;
 	P.nGhost = sm.nGhost;
-	if (order == ORDER_MORTON) {
-		P.new2old = mortonOrder(sm);
-	} else if (order == ORDER_BISECT) {
-		P.new2old = bisectOrder(sm, cellsPerBlock);
-		if (haloGrouping()) groupHaloSources(sm, cellsPerBlock, P.new2old);
-	} else {
-		P.new2old.resize(nE);
-		std::iota(P.new2old.begin(), P.new2old.end(), 0);
-	}
+	P.new2old = cellOrder(sm, order, cellsPerBlock);
 	P.old2new.assign(nE, -1);
 	for (int i = 0; i < nE; i++) P.old2new[P.new2old[i]] = i;
 	auto dev = [&](int cell) { return cell < nE ? P.old2new[cell] : cell; };

@@ -53,5 +53,8 @@ struct HostPlan {
 // exchange: the faces of partition interfaces read the ghost that the interface exchange fills
 // (slot kind 4) instead of the other partition's cell, and the plan carries that exchange
 HostPlan buildPlan(const SolverMesh &sm, int order, int cellsPerBlock, bool exchange = false);
+// the device order of the interior cells (new2old) that buildPlan uses; the cells of each
+// partition of a composite mesh keep their range [partStart[r], partStart[r + 1])
+std::vector<int> cellOrder(const SolverMesh &sm, int order, int cellsPerBlock);
 
 }// namespace e3d

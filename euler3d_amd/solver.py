@@ -24,7 +24,7 @@ STATE_FIELDS = ("rho", "u", "v", "w", "p", "E", "H", "M")
 I32_ARRAYS = ("face2element", "face2nodeStart", "face2node", "element2faceStart", "element2face",
               "element2element", "wallGhost", "wallCell", "wallFace", "farGhost", "farCell", "farFace",
               "symGhost", "symCell", "symFace", "ghostKind", "elemVtk", "elemStart", "elemNodes", "mpiGhost",
-              "mpiCell", "partStart", "haloStart", "haloFrom", "haloTo", "haloSrc", "haloDst")
+              "mpiCell", "partStart", "haloStart", "haloFrom", "haloTo", "haloSrc", "haloDst", "deviceOrder")
 F64_ARRAYS = ("faceNormals", "faceNormalsUnit", "faceSurfaces", "faceCenters", "cellVolumes",
               "cellCentroids", "xyz")
 
@@ -136,7 +136,8 @@ class Mesh:
         raise KeyError(name)
 
     def arrays(self) -> dict:
-        return {k: self.get(k) for k in I32_ARRAYS + F64_ARRAYS}
+        """The mesh's own arrays (the planned deviceOrder is computed on request, not included)."""
+        return {k: self.get(k) for k in I32_ARRAYS + F64_ARRAYS if k != "deviceOrder"}
 
     def close(self) -> None:
         if self._h:

@@ -163,7 +163,8 @@ int e3d_mesh_get_info(const e3d_mesh *m, e3d_mesh_info *out);
  *        and of several partitions read together the interface exchange (MPIHandler::
  *        updateFlowField, one message per sending/receiving partition pair): haloStart (message
  *        offsets), haloFrom, haloTo (its partitions), haloSrc (sent cells), haloDst (receiving
- *        ghosts)
+ *        ghosts), and deviceOrder (the interior cells in the order the solver's default plan
+ *        lays them out on the device: reference cell id of each device position)
  * f64:   faceNormals, faceNormalsUnit, faceSurfaces, faceCenters, cellVolumes, cellCentroids, xyz */
 int e3d_mesh_get_i32(const e3d_mesh *m, const char *name, int *out, long long capacity, long long *n);
 int e3d_mesh_get_f64(const e3d_mesh *m, const char *name, double *out, long long capacity, long long *n);

@@ -50,6 +50,23 @@ def to_local(parts, n_elem, cell):
     raise AssertionError(cell)
 
 
+@pytest.mark.parametrize("case", PART_CASES + ("bump_small_p4_smooth", "wing_small_p2_euler_smooth"))
+def test_device_order_keeps_partition_ranges(case, tmp_path):
+    """The solver's device cell order (bisection blocks, cells read by other blocks grouped last in
+    each block) is a permutation that leaves every partition in its own range of device positions:
+    the per-partition smoothing decisions and coefficient sums reduce over those ranges. With
+    288-cell partitions some 256-cell blocks straddle two partitions."""
+    m = composite(case, tmp_path)
+    parts = m.partitions()
+    order = m.get("deviceOrder")
+    n = m.info()["n_elem"]
+    assert np.array_equal(np.sort(order), np.arange(n))
+    for q in parts:
+        lo, hi = q["elem0"], q["elem0"] + q["n_elem"]
+        seg = order[lo:hi]
+        assert seg.min() >= lo and seg.max() < hi, (case, q)
+
+
 @pytest.mark.parametrize("case", PART_CASES)
 def test_partitions_numbered_as_each_rank(case, tmp_path):
     g = load_golden(case)
