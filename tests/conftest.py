@@ -13,9 +13,10 @@ ROOT = Path(__file__).resolve().parent.parent
 GOLDEN = ROOT / "tests" / "golden"
 sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "oracle"))
+sys.path.insert(0, str(GOLDEN))
 
 FIELDS = ("rho", "u", "v", "w", "p", "E", "H", "M")
-SMALL_CASES = ("block8", "naca9x9x5", "naca_small", "bump_small", "wing_small")
+SMALL_CASES = ("block8", "naca9x9x5", "naca_small", "bump_small", "wing_small", "hybrid_small")
 # run-mode fixtures -> the fixture whose mesh they use: the other integrators, then Run's stopping
 # rules and sampling period (MIN_RESIDUAL, MIN_AEROCOEFF_ERROR, SAMPLING)
 INTEGRATOR_CASES = {"naca_euler": "naca_small", "naca_implicit": "naca_small", "bump_smooth": "bump_small",
@@ -84,6 +85,22 @@ def native_params(pd: dict, **over):
               time_integration=pd.get("time_integration", 2), residual_smoothing=pd.get("residual_smoothing", 0))
     kw.update(over)
     return default_params(**kw)
+
+
+def hybrid_mesh(nx: int, ny: int, jitter: float = 0.08, seed: int = 3):
+    """The hybrid mesh of the hybrid_small fixture (hexahedra, pyramids, tetrahedra and wedges in
+    nx x ny columns, tests/golden/make_golden.py:hybrid_text) at another size, read and
+    preprocessed by the product."""
+    import tempfile
+
+    from make_golden import hybrid_text
+
+    from euler3d_amd import Mesh
+
+    with tempfile.TemporaryDirectory() as td:
+        path = Path(td) / "hybrid.par"
+        path.write_text(hybrid_text(nx, ny, jitter, seed))
+        return Mesh.read(str(path), par=True).preprocess(1.0)
 
 
 @pytest.fixture(scope="session")

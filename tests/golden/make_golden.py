@@ -14,6 +14,8 @@ Cases:
               AoA = 1.25 (the 129x129x2 workload's generator at reduced size)
   bump_small  bump channel, 12x4x4 hexes -> 1152 tets, jitter 0.1, permuted numbering, M = 0.5
   wing_small  swept tapered wing, 25x13x9 nodes, M = 0.84, AoA = 3
+  hybrid_small  all four element types of the reference's connectivity (hexahedra, pyramids,
+              tetrahedra, wedges) in one conforming mesh (hybrid_text below), M = 0.5, AoA = 2
   naca_full   the full 129x129x2 O-grid (16,384 hexes), 2000 iterations: residual history,
               coefficients and the final density and pressure (run mode only)
   Run-mode cases of the other integrators on the meshes above (no new mesh files):
@@ -112,6 +114,9 @@ CASES = {
                                                     semispan=1.5, farfield=15.0),
                        mach=0.84, aoa=3.0, cfl=1.5, iters=30, ocl=2, ocd=0, sref=1.17, cref=1.0,
                        ref=(0.25, 0.0, 0.0), seeds=(0,)),
+    "hybrid_small": dict(source="py:hybrid", gen=dict(nx=4, ny=3, jitter=0.08, seed=3),
+                         mach=0.5, aoa=2.0, cfl=1.5, iters=40, ocl=4, ocd=0, sref=12.0, cref=1.0,
+                         ref=(2.0, 1.5, 0.0), seeds=(0, 5)),
     "naca_full": dict(source="gen:naca", gen=dict(ni=129, nj=129, nk=2, thickness=0.12, wall_spacing=1e-3,
                                                    farfield_radius=20.0, dz=0.1),
                       mach=0.8, aoa=1.25, cfl=1.5, iters=2000, ocl=2, ocd=0, sref=0.1, cref=1.0,
@@ -174,8 +179,105 @@ def parse_residuals(path: Path) -> np.ndarray:
     return np.array(rows, dtype=np.float64)
 
 
+def hybrid_text(nx: int, ny: int, jitter: float, seed: int) -> str:
+    """A conforming mesh of unit cubes, nx x ny columns of four cubes each, bottom to top:
+      z 0-1  a hexahedron;
+      z 1-2  five pyramids on the cube's bottom and side faces with their apex at the cube's centre,
+             and the top face's pyramid cut into two tetrahedra along the diagonal (0,0)-(1,1);
+      z 2-3  two wedges, split along the same diagonal plane;
+      z 3-4  six tetrahedra around the diagonal from the cube's (0,0,0) corner to its (1,1,1) corner,
+             whose bottom faces are cut along that diagonal as well.
+    Shared faces have the same nodes on both sides (quads between hexahedra, pyramids and wedges,
+    triangles between wedges, tetrahedra and the cut pyramids). Interior nodes move by up to
+    `jitter` of the spacing (seeded), so quads are not planar; nodes of the box faces stay on them.
+    Markers: wall (z = 0), farfield (x = 0, x = nx, z = 4), sym (y = 0, y = ny)."""
+    rng = np.random.default_rng(seed)
+    nz = 4
+    grid = {}
+    pts = []
+
+    def node(i, j, k):
+        key = ("g", i, j, k)
+        if key not in grid:
+            x = np.array([i, j, k], dtype=np.float64)
+            d = rng.uniform(-jitter, jitter, 3)
+            if i in (0, nx):
+                d[0] = 0.0
+            if j in (0, ny):
+                d[1] = 0.0
+            if k in (0, nz):
+                d[2] = 0.0
+            grid[key] = len(pts)
+            pts.append(x + d)
+        return grid[key]
+
+    elems = []   # (vtk, nodes)
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                v = [node(i, j, k), node(i + 1, j, k), node(i + 1, j + 1, k), node(i, j + 1, k),
+                     node(i, j, k + 1), node(i + 1, j, k + 1), node(i + 1, j + 1, k + 1), node(i, j + 1, k + 1)]
+                if k == 0:
+                    elems.append((12, v))
+                elif k == 1:
+                    c = len(pts)
+                    pts.append(np.mean([pts[q] for q in v], axis=0))
+                    # bases ordered around the outward normal's opposite (the apex side)
+                    for base in ((v[0], v[3], v[2], v[1]), (v[0], v[1], v[5], v[4]), (v[1], v[2], v[6], v[5]),
+                                 (v[2], v[3], v[7], v[6]), (v[3], v[0], v[4], v[7])):
+                        elems.append((14, [*base, c]))
+                    elems.append((10, [v[4], v[5], v[6], c]))
+                    elems.append((10, [v[4], v[6], v[7], c]))
+                elif k == 2:
+                    elems.append((13, [v[0], v[1], v[2], v[4], v[5], v[6]]))
+                    elems.append((13, [v[0], v[2], v[3], v[4], v[6], v[7]]))
+                else:
+                    for a, b in ((1, 2), (2, 3), (3, 7), (7, 4), (4, 5), (5, 1)):
+                        elems.append((10, [v[0], v[a], v[b], v[6]]))
+    # boundary faces: the cells' faces that lie on the box
+    def faces_of(vtk, n):
+        if vtk == 12:
+            return [(n[0], n[1], n[2], n[3]), (n[4], n[5], n[6], n[7]), (n[0], n[1], n[5], n[4]),
+                    (n[1], n[2], n[6], n[5]), (n[2], n[3], n[7], n[6]), (n[3], n[0], n[4], n[7])]
+        if vtk == 14:
+            return [(n[0], n[1], n[2], n[3]), (n[0], n[1], n[4]), (n[1], n[2], n[4]), (n[2], n[3], n[4]),
+                    (n[3], n[0], n[4])]
+        if vtk == 13:
+            return [(n[0], n[1], n[2]), (n[3], n[4], n[5]), (n[0], n[1], n[4], n[3]), (n[1], n[2], n[5], n[4]),
+                    (n[2], n[0], n[3], n[5])]
+        return [(n[0], n[1], n[2]), (n[0], n[1], n[3]), (n[1], n[2], n[3]), (n[0], n[2], n[3])]
+
+    P = np.array(pts)
+    marks = {"wall": [], "farfield": [], "sym": []}
+    for vtk, n in elems:
+        for f in faces_of(vtk, n):
+            x = P[list(f)]
+            if np.all(x[:, 2] == 0.0):
+                marks["wall"].append(f)
+            elif np.all(x[:, 2] == nz) or np.all(x[:, 0] == 0.0) or np.all(x[:, 0] == nx):
+                marks["farfield"].append(f)
+            elif np.all(x[:, 1] == 0.0) or np.all(x[:, 1] == ny):
+                marks["sym"].append(f)
+    out = ["NDIME= 3", f"NELEM= {len(elems)}"]
+    for e, (vtk, n) in enumerate(elems):
+        out.append(" ".join([str(vtk), *map(str, n), str(e)]))
+    out.append(f"NPOIN= {len(P)}")
+    for q, x in enumerate(P):
+        out.append(f"{x[0]:.17g} {x[1]:.17g} {x[2]:.17g} {q}")
+    out.append("NMARK= 3")
+    for tag in ("wall", "farfield", "sym"):
+        out.append(f"MARKER_TAG= {tag}")
+        out.append(f"MARKER_ELEMS= {len(marks[tag])}")
+        for f in marks[tag]:
+            out.append(" ".join([str(9 if len(f) == 4 else 5), *map(str, f)]))
+    out.append("NINTERNAL_MARKER= 0")
+    return "\n".join(out) + "\n"
+
+
 def mesh_text(case: dict) -> str:
     src = case["source"]
+    if src.startswith("py:"):
+        return {"hybrid": hybrid_text}[src[3:]](**case["gen"])
     if src.startswith("ref:"):
         text = (REF / src[4:]).read_text()
         for a, b in case.get("retag", {}).items():

@@ -98,6 +98,19 @@ def test_mixed_close_to_fp64_wing_and_bump():
     compare("bump", bm, fp64_reference(bm, pb, 200), mixed_run(bm, pb, 200))
 
 
+def test_mixed_close_to_fp64_hybrid():
+    """Cells of 4, 5 and 6 faces (tetrahedra, pyramids, wedges, hexahedra) in the same blocks."""
+    from conftest import hybrid_mesh
+
+    say("[gpu] mixed vs fp64 hybrid 16x12 columns")
+    m = hybrid_mesh(16, 12)
+    prm = default_params(mach=0.5, aoa_deg=2.0, gamma=1.4, cfl=1.5, sref=192.0, cref=1.0, orient_cl=(2, 1),
+                         orient_cd=(0, 1), orient_cm=(1, 1))
+    for scheme in (0, 1):
+        prm.scheme = scheme
+        compare(f"hybrid scheme={scheme}", m, fp64_reference(m, prm, 200), mixed_run(m, prm, 200))
+
+
 def test_uniform_flow_is_preserved_exactly():
     say("[gpu] freestream box stays freestream")
     m = Mesh.box(4, 3, 2, tags=("farfield",) * 6).preprocess(1.0)

@@ -32,8 +32,8 @@ def summary_values(text: str) -> dict:
     return out
 
 
-@pytest.mark.parametrize("case", ["naca_small", "bump_small", "naca_euler", "wing_euler_smooth", "naca_minres",
-                                  "naca_coeffconv", "bump_sampling7"])
+@pytest.mark.parametrize("case", ["naca_small", "bump_small", "hybrid_small", "naca_euler", "wing_euler_smooth",
+                                  "naca_minres", "naca_coeffconv", "bump_sampling7"])
 def test_cli_matches_reference_outputs(case, tmp_path):
     """Also Run's stopping rules and SAMPLING > 1: cp_part_0.dat holds the cp of the last sampled
     iteration, byte for byte as the reference's PrintCp writes it."""

@@ -3,7 +3,7 @@
 The pipelined kernel runs by default only where a block's LDS leaves room for one workgroup per CU
 (hexahedral meshes with large halos, such as the full-size wing, whose state the full-size test
 compares bit for bit with the oracle). Here E3D_PIPE forces it on and off for every plan it fits,
-on tetrahedral, hexahedral and O-grid meshes, Roe and AUSM, with both XCD orders, from a seeded
+on tetrahedral, hexahedral, O-grid and hybrid (all four element types) meshes, Roe and AUSM, with both XCD orders, from a seeded
 (non-uniform) state and from freestream: states and RMS sums after several RK5 iterations must be
 the same bits. The hexahedral wing's plan fits one workgroup per CU, so its one-block-per-workgroup
 runs also cover the 768- and 512-thread forms of that kernel. The iterations cross the first stage
@@ -15,6 +15,8 @@ import sys
 
 import numpy as np
 import pytest
+
+from conftest import hybrid_mesh
 
 from euler3d_amd import EulerSolver, Mesh, default_params
 
@@ -38,6 +40,8 @@ MESHES = {
                                farfield=15.0), dict(mach=0.84, aoa_deg=3.0)),
     "naca": lambda: (Mesh.naca(ni=129, nj=65, nk=2, thickness=0.12, wall_spacing=1e-3, farfield_radius=20.0,
                                dz=0.1), dict(mach=0.8, aoa_deg=1.25)),
+    # hexahedra, pyramids, tetrahedra and wedges: cells of 4, 5 and 6 faces in one block
+    "hybrid": lambda: (hybrid_mesh(16, 12), dict(mach=0.5, aoa_deg=2.0)),
 }
 
 
@@ -57,7 +61,8 @@ def run(monkeypatch, mesh, prm, pipe: int, xcds: int, start=None, iters=7, wide_
 @pytest.mark.parametrize("scheme", [0, 1])
 def test_gpu_pipelined_stage_same_bits(monkeypatch, name, scheme):
     m, pd = MESHES[name]()
-    m.preprocess(1.0)
+    if name != "hybrid":
+        m.preprocess(1.0)
     prm = default_params(cfl=1.5, scheme=scheme, **pd)
     say(f"[gpu] pipelined stage {name} ({m.info()['n_elem']} cells) scheme={scheme}")
     # a non-uniform start: a few iterations from freestream on the default path
