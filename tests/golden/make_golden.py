@@ -38,6 +38,7 @@ Cases:
                               partitioner (recursive coordinate bisection, precision 16)
   bump_small_p4_smooth        bump_small_p4 partitions, RK5 with RESIDUAL_SMOOTHING= 1, 10 iterations
   wing_small_p2_euler_smooth  wing_small in 2 partitions, explicit Euler with smoothing, 20 iterations
+  hybrid_small_p3             hybrid_small in 3 partitions (interfaces between cells of different types)
   Each partitioned case stores its partition files as <case>.p<r>.par.gz.
 
 Usage: python tests/golden/make_golden.py [case ...]
@@ -144,6 +145,7 @@ PART_CASES = {
                                  seeds=(), run_mode=True),
     "wing_small_p2_euler_smooth": dict(CASES["wing_small"], parts=2, mesh_of="wing_small", ti=0, smooth=1, iters=20,
                                        seeds=(), run_mode=True),
+    "hybrid_small_p3": dict(CASES["hybrid_small"], parts=3, mesh_of="hybrid_small", seeds=(0,)),
 }
 MPIEXEC = Path("/opt/conda/bin/mpiexec")
 

@@ -89,7 +89,8 @@ def test_cli_matches_reference_outputs(case, tmp_path):
 
 
 @pytest.mark.parametrize("exchange", ["direct", "halo"])
-@pytest.mark.parametrize("case", ["block8_p4", "naca_small_p3", "bump_small_p4_smooth", "wing_small_p2_euler_smooth"])
+@pytest.mark.parametrize("case", ["block8_p4", "naca_small_p3", "bump_small_p4_smooth", "wing_small_p2_euler_smooth",
+                                  "hybrid_small_p3"])
 def test_cli_partitioned_matches_reference_outputs(case, exchange, tmp_path, monkeypatch):
     """PARTITION_FILES= N: every partition resident on the one device, outputs as the reference's
     `mpiexec -np N` run writes them (residuals.dat, summary.txt with Ncores = N, per-partition

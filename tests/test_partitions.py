@@ -18,7 +18,7 @@ from conftest import GOLDEN, MESH_KEYS_F, load_golden
 
 from euler3d_amd import Mesh
 
-PART_CASES = ("block8_p4", "naca_small_p3", "bump_small_p4")
+PART_CASES = ("block8_p4", "naca_small_p3", "bump_small_p4", "hybrid_small_p3")
 BND_MPI = 4
 
 
