@@ -10,6 +10,7 @@
 // every step is either a map over faces, cells or blocks into disjoint outputs, or a per-block
 // pass whose results are stitched by prefix sums. The plan does not depend on the thread count.
 #include "plan.hpp"
+#include "threads.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -50,21 +51,10 @@ void parallelFor(size_t n, size_t grain, F &&f, int maxT = 0) {
 		f((size_t) 0, n);
 		return;
 	}
-	std::vector<std::exception_ptr> err(T);
-	std::vector<std::thread> th;
-	th.reserve(T - 1);
-	auto run = [&](size_t t) {
-		try {
-			f(n * t / T, n * (t + 1) / T);
-		} catch (...) {
-			err[t] = std::current_exception();
-		}
-	};
-	for (size_t t = 1; t < T; t++) th.emplace_back(run, t);
-	run(0);
-	for (auto &x : th) x.join();
-	for (auto &e : err)
-		if (e) std::rethrow_exception(e);
+	std::vector<std::function<void()>> tasks;
+	tasks.reserve(T);
+	for (size_t t = 0; t < T; t++) tasks.emplace_back([&, t] { f(n * t / T, n * (t + 1) / T); });
+	runAll(tasks);
 }
 
 // sort under a strict total order: sorted runs on several threads, then pairwise merges
@@ -248,10 +238,7 @@ std::vector<int> bisectOrder(const SolverMesh &sm, int cellsPerBlock) {
 			pivot[ax] = kv[ax][half].id;
 		};
 		if (hi - lo > (1 << 15) && depth < 3) {
-			std::thread t1(select, 1), t2(select, 2);
-			select(0);
-			t1.join();
-			t2.join();
+			runAll({[&] { select(0); }, [&] { select(1); }, [&] { select(2); }});
 		} else {
 			for (int ax = 0; ax < 3; ax++) select(ax);
 		}
@@ -282,35 +269,15 @@ std::vector<int> bisectOrder(const SolverMesh &sm, int cellsPerBlock) {
 		for (int i = lo; i < hi; i++) ids[i] = kv[best][i - lo].id;
 		for (auto &v : kv) std::vector<KV>().swap(v);
 		if (depth < 6 && hi - lo > (1 << 14)) {
-			std::exception_ptr err;
-			std::thread left([&, lo, cut, depth] {
-				try {
-					run(lo, cut, depth + 1);
-				} catch (...) {
-					err = std::current_exception();
-				}
-			});
-			run(cut, hi, depth + 1);
-			left.join();
-			if (err) std::rethrow_exception(err);
+			runAll({[&, cut, hi, depth] { run(cut, hi, depth + 1); }, [&, lo, cut, depth] { run(lo, cut, depth + 1); }});
 		} else {
 			run(lo, cut, depth + 1);
 			run(cut, hi, depth + 1);
 		}
 	};
-	std::vector<std::thread> parts;
-	std::vector<std::exception_ptr> perr(bounds.size());
-	for (size_t r = 0; r + 1 < bounds.size(); r++)
-		parts.emplace_back([&, r] {
-			try {
-				run(bounds[r], bounds[r + 1], 0);
-			} catch (...) {
-				perr[r] = std::current_exception();
-			}
-		});
-	for (auto &t : parts) t.join();
-	for (auto &e : perr)
-		if (e) std::rethrow_exception(e);
+	std::vector<std::function<void()>> parts;
+	for (size_t r = 0; r + 1 < bounds.size(); r++) parts.emplace_back([&, r] { run(bounds[r], bounds[r + 1], 0); });
+	runAll(parts);
 	parallelFor((size_t) n, 1 << 15, [&](size_t a, size_t b) {
 		for (size_t i = a; i < b; i++) ids[i] = morton[ids[i]];
 	});

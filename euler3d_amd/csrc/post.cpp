@@ -19,6 +19,7 @@
 // blocks of lines and written in order, so the bytes are identical and multi-million-cell
 // partitions write at disk speed.
 #include "e3d_host.hpp"
+#include "threads.hpp"
 
 #include <algorithm>
 #include <cstdio>
@@ -48,14 +49,14 @@ void parallelFor(size_t n, const std::function<void(size_t, size_t)> &f) {
 		f(0, n);
 		return;
 	}
-	std::vector<std::thread> th;
+	std::vector<std::function<void()>> tasks;
 	size_t per = (n + t - 1) / t;
 	for (int k = 0; k < t; k++) {
 		size_t a = k * per, b = std::min(n, a + per);
 		if (a >= b) break;
-		th.emplace_back(f, a, b);
+		tasks.emplace_back([&f, a, b] { f(a, b); });
 	}
-	for (auto &x : th) x.join();
+	runAll(tasks);
 }
 
 struct Out {
@@ -74,7 +75,6 @@ void writeLines(Out &o, size_t n, const std::function<int(size_t, char *)> &line
 	const int t = workerCount();
 	std::vector<std::string> blk(t);
 	for (size_t base = 0; base < n; base += kBlock * t) {
-		std::vector<std::thread> th;
 		auto fmt = [&](int k) {
 			size_t a = base + k * kBlock, b = std::min(n, a + kBlock);
 			std::string &s = blk[k];
@@ -86,9 +86,9 @@ void writeLines(Out &o, size_t n, const std::function<int(size_t, char *)> &line
 			s.resize(p - s.data());
 		};
 		int used = (int) std::min<size_t>(t, (n - base + kBlock - 1) / kBlock);
-		for (int k = 1; k < used; k++) th.emplace_back(fmt, k);
-		fmt(0);
-		for (auto &x : th) x.join();
+		std::vector<std::function<void()>> tasks;
+		for (int k = 0; k < used; k++) tasks.emplace_back([&fmt, k] { fmt(k); });
+		runAll(tasks);
 		for (int k = 0; k < used; k++) o.put(blk[k].data(), blk[k].size());
 	}
 }
