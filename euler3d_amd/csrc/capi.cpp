@@ -587,6 +587,7 @@ int e3d_mesh_get_info(const e3d_mesh *m, e3d_mesh_info *out) {
 
 int e3d_mesh_get_i32(const e3d_mesh *m, const char *name, int *out, long long capacity, long long *n) {
 	if (!m || !name) return fail(E3D_ERR_ARG, "null argument");
+	if (out && capacity < 0) return fail(E3D_ERR_ARG, "negative capacity");
 	const std::vector<int> *v = nullptr;
 	std::string s(name);
 	const auto &sm = m->sm;
@@ -624,12 +625,14 @@ int e3d_mesh_get_i32(const e3d_mesh *m, const char *name, int *out, long long ca
 	else if (s == "elemNodes") v = &m->file.elemNodes;
 	else return fail(E3D_ERR_ARG, "unknown int32 array " + s);
 	if (n) *n = (long long) v->size();
-	if (out) std::memcpy(out, v->data(), sizeof(int) * (size_t) std::min<long long>(capacity, (long long) v->size()));
+	if (out && !v->empty())
+		std::memcpy(out, v->data(), sizeof(int) * (size_t) std::min<long long>(capacity, (long long) v->size()));
 	return E3D_OK;
 }
 
 int e3d_mesh_get_f64(const e3d_mesh *m, const char *name, double *out, long long capacity, long long *n) {
 	if (!m || !name) return fail(E3D_ERR_ARG, "null argument");
+	if (out && capacity < 0) return fail(E3D_ERR_ARG, "negative capacity");
 	const std::vector<double> *v = nullptr;
 	std::string s(name);
 	const auto &sm = m->sm;
@@ -642,7 +645,8 @@ int e3d_mesh_get_f64(const e3d_mesh *m, const char *name, double *out, long long
 	else if (s == "xyz") v = &m->file.xyz;
 	else return fail(E3D_ERR_ARG, "unknown f64 array " + s);
 	if (n) *n = (long long) v->size();
-	if (out) std::memcpy(out, v->data(), sizeof(double) * (size_t) std::min<long long>(capacity, (long long) v->size()));
+	if (out && !v->empty())
+		std::memcpy(out, v->data(), sizeof(double) * (size_t) std::min<long long>(capacity, (long long) v->size()));
 	return E3D_OK;
 }
 

@@ -64,5 +64,6 @@ def test_batch_case_bytes_and_argument_checks():
     h = ctypes.c_void_p()
     assert lib.e3d_batch_create(0, ctypes.byref(o), ctypes.byref(h)) == 1
     assert "max_cases" in lib.e3d_last_error().decode()
+    raw = Mesh.box(2, 2, 2)  # not preprocessed; kept alive across the call that reads it
     with pytest.raises(native.E3DError):
-        native.check(lib.e3d_batch_case_bytes(Mesh.box(2, 2, 2)._h, None, ctypes.byref(ctypes.c_double())))
+        native.check(lib.e3d_batch_case_bytes(raw._h, None, ctypes.byref(ctypes.c_double())))
