@@ -113,6 +113,39 @@ def build_all(jobs: int = 8) -> None:
     build_reference()
 
 
+def build_sanitized(jobs: int = 8) -> Path:
+    """The library with its host C++ under AddressSanitizer and UBSan (device code as built by
+    build_native, whose objects it links), into _build/asan/ for the CPU tests:
+
+        python -m euler3d_amd.build --sanitize
+        E3D_LIB=_build/asan/libe3d_hip.so ASAN_OPTIONS=detect_leaks=0 \\
+          LD_PRELOAD="$(gcc -print-file-name=libasan.so) $(gcc -print-file-name=libstdc++.so.6)" \\
+          python -m pytest tests -m "not gpu"
+
+    (libstdc++ is preloaded with the sanitizer runtime so that it can intercept C++ throws in a
+    Python process.)"""
+    out = ROOT / "_build" / "asan"
+    out.mkdir(parents=True, exist_ok=True)
+    flags = [f for f in CXXFLAGS if f != "-O2"] + ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined"]
+    steps, objs = [], []
+    for s in HOST_SRCS:
+        o = out / (s + ".o")
+        steps.append(["g++", *flags, "-c", str(CSRC / s), "-o", str(o)])
+        objs.append(o)
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        list(ex.map(_run, steps))
+    dev = [OBJ / (s + ".o") for s in DEVICE_SRCS]
+    if not all(o.exists() for o in dev):
+        build_native(jobs)
+    lib = out / "libe3d_hip.so"
+    _run(["g++", "-shared", "-fsanitize=address,undefined", *map(str, objs), *map(str, dev), "-o", str(lib),
+          f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"])
+    return lib
+
+
 if __name__ == "__main__":
-    build_all()
-    print("built", PKG / "lib" / "libe3d_hip.so")
+    if "--sanitize" in sys.argv[1:]:
+        print("built", build_sanitized())
+    else:
+        build_all()
+        print("built", PKG / "lib" / "libe3d_hip.so")

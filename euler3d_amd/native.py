@@ -11,7 +11,9 @@ from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "lib" / "libe3d_hip.so"
+# E3D_LIB selects another build of the same library (the sanitized host build of the tests)
+LIB_PATH = Path(os.environ["E3D_LIB"]).resolve() if os.environ.get("E3D_LIB") else \
+    Path(__file__).resolve().parent / "lib" / "libe3d_hip.so"
 
 E3D_OK = 0
 ERRORS = {1: "argument", 2: "io", 3: "mesh", 4: "device", 5: "unsupported", 6: "busy", 7: "numeric"}
